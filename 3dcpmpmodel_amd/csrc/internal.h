@@ -1,0 +1,85 @@
+// internal.h -- context and launcher declarations shared by the library's
+// translation units.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/cpmp.h"
+
+namespace cpmp {
+
+// Parameters as the kernels see them (one kernel argument; replaces the
+// __constant__ symbols of FluidCuda.cu:67 and Common.h:96-153).
+struct DevParams {
+    int nx, ny, nz;
+    int lo, hi;                 // solid layers at the z ends
+    long long N;
+    int eos, mu_form, lap_points, wall_scheme;
+    int qp_on;                  // K1 or K2 non-zero
+    double A, B, T, K2s, Ka;    // K2s = K*K
+    double K1, K2;
+    double ts[19];
+    double rho0;                // FP16 shift density
+};
+
+enum Kern { KW = CPMP_K_WALL, KM = CPMP_K_MU, KC = CPMP_K_COLLIDE, KR = CPMP_K_MACRO };
+
+struct Timers {
+    bool on = false;
+    hipEvent_t ev[2 * CPMP_K_COUNT * 8] = {};
+    double ms[CPMP_K_COUNT] = {};
+    long long count[CPMP_K_COUNT] = {};
+};
+
+}  // namespace cpmp
+
+struct cpmp_ctx {
+    cpmp_params p;
+    cpmp::DevParams dp;
+    int64_t N = 0, NXY = 0;
+    int form = 0;
+    size_t pop_bytes = 0;       // bytes of one population value
+    size_t scal_bytes = 0;      // bytes of one rho/mu value
+    hipStream_t stream = nullptr;
+    void *f[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *rho = nullptr, *mu = nullptr;
+    double *mv = nullptr;       // 3*N, MV of the last step
+    double *mu_w = nullptr;     // 2*NXY: bottom then top
+    double *red = nullptr;      // reduction partials
+    double *staging = nullptr;  // N doubles
+    int64_t step = 0;
+    bool mv_valid = false;
+    cpmp::Timers *timers = nullptr;
+};
+
+namespace cpmp {
+
+int set_error(int code, const char *fmt, ...);
+
+// launchers (kernels.hip); all enqueue on ctx->stream
+hipError_t launch_wall(cpmp_ctx *c);
+hipError_t launch_mu(cpmp_ctx *c);
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv);
+hipError_t launch_macro(cpmp_ctx *c);
+hipError_t launch_step(cpmp_ctx *c, bool want_mv);
+hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, double rg, double rl, double solid);
+hipError_t launch_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d);
+hipError_t launch_feq_from_rho(cpmp_ctx *c);
+hipError_t launch_pop_from_double(cpmp_ctx *c, int q, const double *src, void *dst_pop);
+hipError_t launch_pop_to_double(cpmp_ctx *c, int q, const void *src_pop, double *dst);
+hipError_t launch_scalar_from_double(cpmp_ctx *c, const double *src, void *dst);
+hipError_t launch_scalar_to_double(cpmp_ctx *c, const void *src, double *dst);
+hipError_t launch_velocity(cpmp_ctx *c, double *out3);
+hipError_t launch_force(cpmp_ctx *c, double *out3);
+hipError_t launch_reduce(cpmp_ctx *c, double *host_out /* 12 */);
+hipError_t launch_type(cpmp_ctx *c, double *out);
+size_t reduce_scratch_doubles();
+
+// host-side CPU evaluation of the per-node collision (self-test hook)
+void host_collide_fp64(const double f[19], double rho, const double u[3], const double F[3],
+                       const double ts[19], const double *qp, double out[19]);
+
+}  // namespace cpmp

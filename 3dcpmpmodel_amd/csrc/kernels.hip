@@ -1,0 +1,611 @@
+// kernels.hip -- gfx950 kernels of the chemical-potential multiphase D3Q19 MRT
+// step and their launchers.
+//
+// Layout: reference order I(x,y,z) = (x*ny + y)*nz + z (Common.h:164), z fastest,
+// populations SoA f[q*N + I].  Node types are implicit: solid iff z lies in the
+// wall layers (the geometry of SetFlowField, FluidCuda.cu:204), so no type array
+// is read.  Storage forms: FP64 (double f, rho, mu), FP32 (float), FP16
+// (f stored as f - w_q*rho0 in _Float16; rho, mu float).
+#include "internal.h"
+#include "lbm_math.h"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace cpmp {
+
+namespace {
+
+template <class TF_, class TS_, class TC_>
+struct Form { using TF = TF_; using TS = TS_; using TC = TC_; };
+using FormF64 = Form<double, double, double>;
+using FormF32 = Form<float, float, float>;
+using FormF16 = Form<_Float16, float, float>;
+
+template <class TF> constexpr bool kShifted = std::is_same<TF, _Float16>::value;
+
+template <class Fn>
+hipError_t dispatch(int form, Fn &&fn) {
+    switch (form) {
+    case CPMP_FORM_FP64: return fn(FormF64{});
+    case CPMP_FORM_FP32: return fn(FormF32{});
+    case CPMP_FORM_FP16: return fn(FormF16{});
+    }
+    return hipErrorInvalidValue;
+}
+
+template <class TF, class TC>
+__device__ __forceinline__ TC pop_load(const TF *p, long long i, int q, double rho0) {
+    if constexpr (kShifted<TF>) return (TC)p[i] + (TC)(wq(q) * rho0);
+    else return (TC)p[i];
+}
+template <class TF, class TC>
+__device__ __forceinline__ TF pop_val(TC v, int q, double rho0) {
+    if constexpr (kShifted<TF>) return (TF)(v - (TC)(wq(q) * rho0));
+    else return (TF)v;
+}
+
+__device__ __forceinline__ bool is_solid(const DevParams &P, int z) { return z < P.lo || z >= P.nz - P.hi; }
+__device__ __forceinline__ int wrapc(int a, int n) { a %= n; return a < 0 ? a + n : a; }
+
+struct Coord { int x, y, z; };
+__device__ __forceinline__ Coord decompose(const DevParams &P, long long I) {
+    Coord c;
+    c.z = (int)(I % P.nz);
+    const long long r = I / P.nz;
+    c.y = (int)(r % P.ny);
+    c.x = (int)(r / P.ny);
+    return c;
+}
+
+constexpr int TPB = 256;
+
+// ---------------------------------------------------------------------------
+// Wall fill, one pass (ChemBoundary, FluidCuda.cu:376-401, as ordered layers).
+// Pass p writes bottom layer lo-1-p and top layer nz-hi+p from the layer next to
+// it (layered scheme) or from the first fluid layer (simple scheme).  Pass 0 also
+// writes the wall chemical potential into every solid node of the column.
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_wall(const DevParams P, TS *__restrict__ rho, TS *__restrict__ mu,
+                                              const double *__restrict__ muw, int pass) {
+    const long long col = (long long)blockIdx.x * TPB + threadIdx.x;
+    const long long nxy = (long long)P.nx * P.ny;
+    if (col >= nxy) return;
+    const int x = (int)(col / P.ny), y = (int)(col % P.ny);
+    const int x1 = wrapc(x - 1, P.nx), x2 = wrapc(x + 1, P.nx), y1 = wrapc(y - 1, P.ny), y2 = wrapc(y + 1, P.ny);
+    auto at = [&](int xx, int yy, int k) { return ((long long)xx * P.ny + yy) * P.nz + k; };
+    auto avg = [&](int k) {
+        return ((double)rho[at(x, y, k)] * 2 + (double)rho[at(x1, y1, k)] + (double)rho[at(x1, y2, k)]
+                + (double)rho[at(x2, y1, k)] + (double)rho[at(x2, y2, k)]) / 6;
+    };
+    if (pass < P.lo) {
+        const int kd = P.lo - 1 - pass, ks = P.wall_scheme ? kd + 1 : P.lo;
+        rho[at(x, y, kd)] = (TS)avg(ks);
+    }
+    if (pass < P.hi) {
+        const int kd = P.nz - P.hi + pass, ks = P.wall_scheme ? kd - 1 : P.nz - 1 - P.hi;
+        rho[at(x, y, kd)] = (TS)avg(ks);
+    }
+    if (pass == 0) {
+        const double ml = muw[col], mh = muw[nxy + col];
+        for (int k = 0; k < P.lo; ++k) mu[at(x, y, k)] = (TS)ml;
+        for (int k = P.nz - P.hi; k < P.nz; ++k) mu[at(x, y, k)] = (TS)mh;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Chemical potential (ChemPotential, FluidCuda.cu:444-463), FP64 arithmetic in
+// every form: mu = K^2 mu_eos(clamp(rho)) - Ka * Laplacian(rho).
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const Coord c = decompose(P, I);
+    if (is_solid(P, c.z)) return;
+    auto at = [&](int dx, int dy, int dz) {
+        return (double)rho[((long long)wrapc(c.x + dx, P.nx) * P.ny + wrapc(c.y + dy, P.ny)) * P.nz + wrapc(c.z + dz, P.nz)];
+    };
+    const double d = (double)rho[I];
+    const double m = mu_eos(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho(d));
+    double kal;
+    if (P.lap_points == 7) {   // GradD7, Common.h:243-246
+        auto ax = [&](int ux, int uy, int uz) {
+            return 2 * at(-3 * ux, -3 * uy, -3 * uz) - 27 * at(-2 * ux, -2 * uy, -2 * uz) + 270 * at(-ux, -uy, -uz)
+                   + 270 * at(ux, uy, uz) - 27 * at(2 * ux, 2 * uy, 2 * uz) + 2 * at(3 * ux, 3 * uy, 3 * uz);
+        };
+        kal = P.Ka * (-d * 490 * 3 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 180;
+    } else {                   // GradD5, Common.h:189-192
+        auto ax = [&](int ux, int uy, int uz) {
+            return -at(-2 * ux, -2 * uy, -2 * uz) + at(-ux, -uy, -uz) * 16 + at(ux, uy, uz) * 16 - at(2 * ux, 2 * uy, 2 * uz);
+        };
+        kal = P.Ka * ((-d * 90 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 12);
+    }
+    mu[I] = (TS)(P.K2s * m - kal);
+}
+
+// ---------------------------------------------------------------------------
+// Isotropic 6th-order gradient of Common.h:222-241 at (x,y,z) with wrapped
+// coordinates xs/ys/zs[s+3] = coordinate + s.
+template <class TS, class TC>
+struct Stencil {
+    const DevParams &P;
+    const int *xs, *ys, *zs;
+    __device__ __forceinline__ TC at(const TS *a, int dx, int dy, int dz) const {
+        return (TC)a[((long long)xs[dx + 3] * P.ny + ys[dy + 3]) * P.nz + zs[dz + 3]];
+    }
+    __device__ __forceinline__ TC dd(const TS *a, int ux, int uy, int uz) const {
+        return -at(a, -3 * ux, -3 * uy, -3 * uz) + TC(9) * at(a, -2 * ux, -2 * uy, -2 * uz)
+               - TC(45) * at(a, -ux, -uy, -uz) + TC(45) * at(a, ux, uy, uz)
+               - TC(9) * at(a, 2 * ux, 2 * uy, 2 * uz) + at(a, 3 * ux, 3 * uy, 3 * uz);
+    }
+    __device__ __forceinline__ void grad(const TS *a, TC &gx, TC &gy, TC &gz) const {
+        const TC s = TC(1.0 / 360.0);
+        gx = (TC(2) * dd(a, 1, 0, 0) + dd(a, 1, 0, 1) + dd(a, 1, 0, -1) + dd(a, 1, 1, 0) + dd(a, 1, -1, 0)) * s;
+        gy = (TC(2) * dd(a, 0, 1, 0) + dd(a, 0, 1, 1) + dd(a, 0, 1, -1) + dd(a, 1, 1, 0) + dd(a, -1, 1, 0)) * s;
+        gz = (TC(2) * dd(a, 0, 0, 1) + dd(a, 1, 0, 1) + dd(a, -1, 0, 1) + dd(a, 0, 1, 1) + dd(a, 0, -1, 1)) * s;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Fused non-ideal force + MRT collision + exact-difference forcing + push
+// streaming with half-way bounce-back (NonidealForce + GlobalCollide,
+// FluidCuda.cu:311-347, 431-441).  u = j(f)/rho with rho the stored density.
+template <class TF, class TS, class TC, bool MV>
+__global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__restrict__ fin, TF *__restrict__ fout,
+                                                 const TS *__restrict__ rho, const TS *__restrict__ mu,
+                                                 double *__restrict__ mv) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const Coord c = decompose(P, I);
+    if (is_solid(P, c.z)) return;
+    const long long N = P.N;
+    int xs[7], ys[7], zs[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+        xs[s] = wrapc(c.x + s - 3, P.nx);
+        ys[s] = wrapc(c.y + s - 3, P.ny);
+        zs[s] = wrapc(c.z + s - 3, P.nz);
+    }
+    const Stencil<TS, TC> st{P, xs, ys, zs};
+
+    TC f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, (long long)q * N + I, q, P.rho0);
+
+    const TC r = (TC)rho[I];
+    const TC inv = TC(1) / r;
+    TC jx, jy, jz;
+    momentum(f, jx, jy, jz);
+    const TC ux = jx * inv, uy = jy * inv, uz = jz * inv;
+
+    TC gmx, gmy, gmz, grx, gry, grz;
+    st.grad(mu, gmx, gmy, gmz);
+    st.grad(rho, grx, gry, grz);
+    const TC third = TC(1.0 / 3.0);
+    const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
+    if (MV) {
+        mv[I] = (double)(ux + Fx * inv * TC(0.5));
+        mv[N + I] = (double)(uy + Fy * inv * TC(0.5));
+        mv[2 * N + I] = (double)(uz + Fz * inv * TC(0.5));
+    }
+
+    TC ts[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ts[q] = (TC)P.ts[q];
+    TC out[Q];
+    if (P.qp_on) {
+        TC qp[Q];
+        qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
+        collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, qp, out);
+    } else {
+        collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, (const TC *)nullptr, out);
+    }
+
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int tz = zs[3 + cz(q)];
+        if (is_solid(P, tz)) {
+            fout[(long long)opp(q) * N + I] = pop_val<TF, TC>(out[q], opp(q), P.rho0);
+        } else {
+            const long long T = ((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz;
+            fout[(long long)q * N + T] = pop_val<TF, TC>(out[q], q, P.rho0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Density (MacroCalculate, FluidCuda.cu:351-372): rho = sum_q f_q on fluid nodes.
+template <class TF, class TS, class TC>
+__global__ void __launch_bounds__(TPB) k_macro(const DevParams P, const TF *__restrict__ f, TS *__restrict__ rho) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    if (is_solid(P, (int)(I % P.nz))) return;
+    TC s = TC(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) s += pop_load<TF, TC>(f, (long long)q * P.N + I, q, P.rho0);
+    rho[I] = (TS)s;
+}
+
+// velocity u = j/rho, rho = sum f (fluid nodes; 0 on solid)
+template <class TF, class TC>
+__global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *__restrict__ fp, double *__restrict__ out) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    if (is_solid(P, (int)(I % P.nz))) { out[I] = out[P.N + I] = out[2 * P.N + I] = 0; return; }
+    TC f[Q], s = TC(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) { f[q] = pop_load<TF, TC>(fp, (long long)q * P.N + I, q, P.rho0); s += f[q]; }
+    TC jx, jy, jz;
+    momentum(f, jx, jy, jz);
+    out[I] = (double)(jx / s);
+    out[P.N + I] = (double)(jy / s);
+    out[2 * P.N + I] = (double)(jz / s);
+}
+
+// force field F = -rho grad(mu) + grad(rho)/3 (NonidealForce) into 3N doubles
+template <class TS, class TC>
+__global__ void __launch_bounds__(TPB) k_force(const DevParams P, const TS *__restrict__ rho, const TS *__restrict__ mu,
+                                               double *__restrict__ out) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const Coord c = decompose(P, I);
+    if (is_solid(P, c.z)) { out[I] = out[P.N + I] = out[2 * P.N + I] = 0; return; }
+    int xs[7], ys[7], zs[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+        xs[s] = wrapc(c.x + s - 3, P.nx); ys[s] = wrapc(c.y + s - 3, P.ny); zs[s] = wrapc(c.z + s - 3, P.nz);
+    }
+    const Stencil<TS, TC> st{P, xs, ys, zs};
+    TC gmx, gmy, gmz, grx, gry, grz;
+    st.grad(mu, gmx, gmy, gmz);
+    st.grad(rho, grx, gry, grz);
+    const TC r = (TC)rho[I], third = TC(1.0 / 3.0);
+    out[I] = (double)(-gmx * r + grx * third);
+    out[P.N + I] = (double)(-gmy * r + gry * third);
+    out[2 * P.N + I] = (double)(-gmz * r + grz * third);
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions (SetFlowField, FluidCuda.cu:199-241)
+template <class TF, class TS>
+__global__ void __launch_bounds__(TPB) k_init_reference(const DevParams P, TF *__restrict__ f, TS *__restrict__ rho,
+                                                        TS *__restrict__ mu, double radius, double width, double rg,
+                                                        double rl, double solid) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const Coord c = decompose(P, I);
+    double d;
+    if (is_solid(P, c.z)) {
+        d = solid;
+        mu[I] = (TS)solid;
+    } else {
+        mu[I] = (TS)0;
+        const double ddx = double(c.x) - P.nx / 2, ddy = double(c.y) - P.ny / 2, ddz = double(c.z) - (radius + 2);
+        const double r = (double)sqrtf((float)(ddx * ddx + ddy * ddy + ddz * ddz));   // FluidCuda.cu:215
+        d = (rl + rg) / 2 - (rl - rg) / 2 * tanh(double(r - radius) * 2 / width);
+    }
+    rho[I] = (TS)d;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[(long long)q * P.N + I] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+}
+
+template <class TF, class TS>
+__global__ void __launch_bounds__(TPB) k_init_spheres(const DevParams P, TF *__restrict__ f, TS *__restrict__ rho,
+                                                      TS *__restrict__ mu, const cpmp_init_desc D) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const Coord c = decompose(P, I);
+    double d;
+    if (is_solid(P, c.z)) {
+        d = D.solid_rho;
+        mu[I] = (TS)D.solid_rho;
+    } else {
+        double phi = 0;
+        for (int s = 0; s < D.nspheres; ++s) {
+            const double dx = c.x - D.cx[s], dy = c.y - D.cy[s], dz = c.z - D.cz[s];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            phi = fmax(phi, 0.5 * (1.0 - tanh(2.0 * (r - D.radius[s]) / D.width)));
+        }
+        if (D.invert) phi = 1.0 - phi;
+        d = D.rho_g + (D.rho_l - D.rho_g) * phi;
+        if (D.noise_amp != 0.0) d += D.noise_amp * noise_u(D.seed, (uint64_t)I);
+        mu[I] = (TS)0;
+    }
+    rho[I] = (TS)d;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[(long long)q * P.N + I] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+}
+
+template <class TF, class TS>
+__global__ void __launch_bounds__(TPB) k_feq_from_rho(const DevParams P, TF *__restrict__ f, const TS *__restrict__ rho) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const double d = (double)rho[I];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[(long long)q * P.N + I] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+}
+
+// ---------------------------------------------------------------------------
+// conversions between the storage forms and FP64 host values
+template <class TF>
+__global__ void __launch_bounds__(TPB) k_pop_from_double(const DevParams P, int q, const double *__restrict__ src,
+                                                         TF *__restrict__ dst) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    dst[(long long)q * P.N + I] = pop_val<TF, double>(src[I], q, P.rho0);
+}
+template <class TF>
+__global__ void __launch_bounds__(TPB) k_pop_to_double(const DevParams P, int q, const TF *__restrict__ src,
+                                                       double *__restrict__ dst) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    dst[I] = pop_load<TF, double>(src, (long long)q * P.N + I, q, P.rho0);
+}
+template <class A, class B>
+__global__ void __launch_bounds__(TPB) k_convert(long long n, const A *__restrict__ src, B *__restrict__ dst) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I < n) dst[I] = (B)src[I];
+}
+__global__ void __launch_bounds__(TPB) k_type(const DevParams P, double *__restrict__ out) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I < P.N) out[I] = is_solid(P, (int)(I % P.nz)) ? 2.0 : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (CalcMacroCPU, FluidCuda.cu:1626-1657) in FP64: fixed grid, fixed
+// per-thread order, wavefront shuffles + LDS, then one block over the partials.
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_VALS = 12;   // mass, bad, first_bad, rmin, rmax, maxspeed2, mom x/y/z, kin, count, -
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// kinds: 0 sum, 1 min, 2 max
+__device__ __forceinline__ void block_reduce(double (&v)[RED_VALS], const int (&kind)[RED_VALS], double *out) {
+    __shared__ double sh[TPB / 64][RED_VALS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < RED_VALS; ++k) {
+        double x = kind[k] == 0 ? wave_sum(v[k]) : kind[k] == 1 ? wave_min(v[k]) : wave_max(v[k]);
+        if (lane == 0) sh[wv][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < RED_VALS; ++k) {
+            double x = sh[0][k];
+            for (int w = 1; w < TPB / 64; ++w)
+                x = kind[k] == 0 ? x + sh[w][k] : kind[k] == 1 ? fmin(x, sh[w][k]) : fmax(x, sh[w][k]);
+            out[k] = x;
+        }
+    }
+}
+
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_reduce(const DevParams P, const TS *__restrict__ rho,
+                                                const double *__restrict__ mv, int has_mv, double *__restrict__ part) {
+    const int kind[RED_VALS] = {0, 0, 1, 1, 2, 2, 0, 0, 0, 0, 0, 0};
+    double v[RED_VALS] = {0, 0, 1e300, 1e300, -1e300, 0, 0, 0, 0, 0, 0, 0};
+    const long long stride = (long long)gridDim.x * TPB;
+    for (long long I = (long long)blockIdx.x * TPB + threadIdx.x; I < P.N; I += stride) {
+        if (is_solid(P, (int)(I % P.nz))) continue;
+        const double d = (double)rho[I];
+        v[0] += d;
+        if (!(d >= 0)) { v[1] += 1; v[2] = fmin(v[2], (double)I); }
+        v[3] = fmin(v[3], d);
+        v[4] = fmax(v[4], d);
+        v[10] += 1;
+        if (has_mv) {
+            const double a = mv[I], b = mv[P.N + I], e = mv[2 * P.N + I];
+            const double m2 = a * a + b * b + e * e;
+            v[5] = fmax(v[5], m2);
+            v[6] += d * a; v[7] += d * b; v[8] += d * e;
+            v[9] += 0.5 * d * m2;
+        }
+    }
+    block_reduce(v, kind, part + (long long)blockIdx.x * RED_VALS);
+}
+__global__ void __launch_bounds__(TPB) k_reduce_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+    const int kind[RED_VALS] = {0, 0, 1, 1, 2, 2, 0, 0, 0, 0, 0, 0};
+    double v[RED_VALS] = {0, 0, 1e300, 1e300, -1e300, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += TPB) {
+#pragma unroll
+        for (int k = 0; k < RED_VALS; ++k) {
+            const double x = part[(long long)b * RED_VALS + k];
+            v[k] = kind[k] == 0 ? v[k] + x : kind[k] == 1 ? fmin(v[k], x) : fmax(v[k], x);
+        }
+    }
+    block_reduce(v, kind, out);
+}
+
+inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+}  // namespace
+
+// ===========================================================================
+// launchers
+hipError_t launch_wall(cpmp_ctx *c) {
+    const DevParams &P = c->dp;
+    if (P.lo == 0 && P.hi == 0) return hipSuccess;
+    const int passes = std::max(P.lo, P.hi);
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        for (int p = 0; p < passes; ++p)
+            hipLaunchKernelGGL(k_wall<TS>, dim3(blocks_for(c->NXY)), dim3(TPB), 0, c->stream, P, (TS *)c->rho,
+                               (TS *)c->mu, c->mu_w, p);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_mu(cpmp_ctx *c) {
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL(k_mu<TS>, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        using TS = typename decltype(fm)::TS;
+        using TC = typename decltype(fm)::TC;
+        const TF *fin = (const TF *)c->f[c->cur];
+        TF *fout = (TF *)c->f[c->cur ^ 1];
+        if (want_mv)
+            hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, fin,
+                               fout, (const TS *)c->rho, (const TS *)c->mu, c->mv);
+        else
+            hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, fin,
+                               fout, (const TS *)c->rho, (const TS *)c->mu, c->mv);
+        c->cur ^= 1;
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_macro(cpmp_ctx *c) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        using TS = typename decltype(fm)::TS;
+        using TC = typename decltype(fm)::TC;
+        hipLaunchKernelGGL((k_macro<TF, TS, TC>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                           (const TF *)c->f[c->cur], (TS *)c->rho);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_step(cpmp_ctx *c, bool want_mv) {
+    hipError_t e;
+    if ((e = launch_wall(c)) != hipSuccess) return e;
+    if ((e = launch_mu(c)) != hipSuccess) return e;
+    if ((e = launch_collide(c, want_mv)) != hipSuccess) return e;
+    return launch_macro(c);
+}
+
+hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, double rg, double rl, double solid) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_init_reference<TF, TS>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                           (TF *)c->f[c->cur], (TS *)c->rho, (TS *)c->mu, radius, width, rg, rl, solid);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_init_spheres<TF, TS>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                           (TF *)c->f[c->cur], (TS *)c->rho, (TS *)c->mu, *d);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_feq_from_rho(cpmp_ctx *c) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_feq_from_rho<TF, TS>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                           (TF *)c->f[c->cur], (const TS *)c->rho);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_pop_from_double(cpmp_ctx *c, int q, const double *src, void *dst) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        hipLaunchKernelGGL(k_pop_from_double<TF>, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, q, src, (TF *)dst);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_pop_to_double(cpmp_ctx *c, int q, const void *src, double *dst) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        hipLaunchKernelGGL(k_pop_to_double<TF>, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, q,
+                           (const TF *)src, dst);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_scalar_from_double(cpmp_ctx *c, const double *src, void *dst) {
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_convert<double, TS>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, (long long)c->N, src,
+                           (TS *)dst);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_scalar_to_double(cpmp_ctx *c, const void *src, double *dst) {
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_convert<TS, double>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, (long long)c->N,
+                           (const TS *)src, dst);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_velocity(cpmp_ctx *c, double *out3) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        using TC = typename decltype(fm)::TC;
+        hipLaunchKernelGGL((k_velocity<TF, TC>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                           (const TF *)c->f[c->cur], out3);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_force(cpmp_ctx *c, double *out3) {
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        using TC = typename decltype(fm)::TC;
+        hipLaunchKernelGGL((k_force<TS, TC>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
+                           (const TS *)c->mu, out3);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_type(cpmp_ctx *c, double *out) {
+    hipLaunchKernelGGL(k_type, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(cpmp_ctx *c, double *host_out) {
+    hipError_t e = dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL(k_reduce<TS>, dim3(RED_BLOCKS), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, c->mv,
+                           c->mv_valid ? 1 : 0, c->red);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(TPB), 0, c->stream, c->red, RED_BLOCKS, c->red + RED_BLOCKS * RED_VALS);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(host_out, c->red + RED_BLOCKS * RED_VALS, RED_VALS * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream)) != hipSuccess)
+        return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+size_t reduce_scratch_doubles() { return (size_t)(RED_BLOCKS + 1) * RED_VALS; }
+
+// host evaluation of the collision (same code as the device path, FP64)
+void host_collide_fp64(const double f[19], double rho, const double u[3], const double F[3], const double ts[19],
+                       const double *qp, double out[19]) {
+    collide(f, rho, u[0], u[1], u[2], F[0], F[1], F[2], ts, qp, out);
+}
+
+}  // namespace cpmp

@@ -1,0 +1,114 @@
+// observables.cpp -- droplet observables on the host: the reference's
+// contact-angle measurement and a 3-D spherical-cap fit.
+#include "internal.h"
+
+#include <cmath>
+#include <vector>
+
+using namespace cpmp;
+
+namespace {
+
+int fetch_rho(cpmp_ctx *c, std::vector<double> &rho) {
+    rho.resize(c->N);
+    return cpmp_get_field(c, CPMP_FIELD_RHO, rho.data());
+}
+
+}  // namespace
+
+extern "C" {
+
+// ContactAngle(side=0, tier), FluidCuda.cu:557-689.  Interface x on rows z=2..5 by
+// linear interpolation at (rhoG+rhoL)/2 (:595-623), linear extrapolation of rows
+// z=3,4 to z=2 (:634-637), angles from the extrapolated foot (:641-649), and the
+// spherical-cap angle from height H and base L (:656-680).  When no interface
+// crosses the last row scanned for H, H is NaN (the reference leaves it unset).
+int cpmp_contact_angle(cpmp_ctx *c, int tier, double rho_g, double rho_l, double *out) {
+    if (!c || !out) return set_error(CPMP_EINVAL, "bad context or output");
+    const int DX = c->p.nx, DY = c->p.ny, DZ = c->p.nz;
+    if (tier < 0 || tier >= DY) return set_error(CPMP_EINVAL, "tier %d outside [0,%d)", tier, DY);
+    if (DX < 8 || DZ < 8) return set_error(CPMP_EINVAL, "domain too small for the contact-angle scan");
+    std::vector<double> rho;
+    int rc = fetch_rho(c, rho);
+    if (rc) return rc;
+    const double PI = 3.14159265358979323846264338327950288;
+    const double Mid = (rho_g + rho_l) / 2;
+    auto T = [&](int i, int k) { return rho[((int64_t)i * DY + tier) * DZ + k]; };
+    double Px[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = 2; j <= 5; ++j) {
+        const int n = (j - 2) * 2;
+        for (int i = 0; i < DX / 2 + 2; ++i) {
+            const double t1 = T(i + 1, j), t2 = T(i + 2, j);
+            if (t1 < Mid && t2 >= Mid) { Px[n] = double(i + 1) + (t1 - Mid) / (t1 - t2); break; }
+        }
+        for (int i = DX - 2; i > DX / 2 - 2; --i) {
+            const double t1 = T(i - 1, j), t2 = T(i - 2, j);
+            if (t1 < Mid && t2 >= Mid) { Px[n + 1] = double(i - 1) - (t1 - Mid) / (t1 - t2); break; }
+        }
+    }
+    const double y0 = 2, y1 = 3, y2 = 4;
+    double CaLeft = 0, CaRight = 0, CaCap = 0, XLeft = DY / 2, XRight = DY / 2, H = NAN, L = 0;
+    int valid = 0;
+    if (Px[0] != 0.0 && Px[1] != 0.0) {
+        valid = 1;
+        XLeft = Px[2] * ((y0 - y2) / (y1 - y2)) + Px[4] * ((y0 - y1) / (y2 - y1));
+        XRight = Px[3] * ((y0 - y2) / (y1 - y2)) + Px[5] * ((y0 - y1) / (y2 - y1));
+        double a = std::atan((y1 - y0) / (Px[2] - XLeft));
+        if (a < 0) a += PI;
+        CaLeft = a / PI * 180;
+        a = std::atan(-(y1 - y0) / (Px[3] - XRight));
+        if (a < 0) a += PI;
+        CaRight = a / PI * 180;
+        L = XRight - XLeft;
+        const int ic = int((Px[4] + Px[5]) / 2 + 0.5);
+        if (ic >= 0 && ic < DX)
+            for (int j = 2; j < DZ - 2; ++j) {
+                const double t = T(ic, j), t1 = T(ic, j + 1);
+                if (t >= Mid && t1 < Mid) H = double(j) + (t - Mid) / (t - t1) - y0;
+            }
+        if (L <= 0) { L = Px[3] - Px[2]; H = H - 0.5; }
+        if (XLeft > DY / 2 && XRight < DY / 2) { XLeft = DY / 2; XRight = DY / 2; }   // as FluidCuda.cu:672
+        const double R = (H * H * 4 + L * L) / H / 8;
+        CaCap = std::atan(L / (R - H) / 2);
+        if (CaCap < 0) CaCap += PI;
+        CaCap = CaCap / PI * 180;
+    }
+    out[0] = CaLeft; out[1] = CaRight; out[2] = CaCap; out[3] = XLeft; out[4] = XRight; out[5] = H; out[6] = L;
+    out[7] = valid;
+    return CPMP_OK;
+}
+
+// 3-D spherical cap on the bottom wall: base area = number of wall-adjacent columns
+// whose first fluid layer is above the interface density, a = sqrt(area/pi);
+// height H = highest interface crossing above the wall plane z = lo - 1/2
+// (half-way bounce-back); R = (a^2+H^2)/(2H); angle = atan2(a, R-H).
+int cpmp_cap_fit(cpmp_ctx *c, double rho_g, double rho_l, double *out) {
+    if (!c || !out) return set_error(CPMP_EINVAL, "bad context or output");
+    if (c->p.nwall_lo < 1) return set_error(CPMP_EINVAL, "no bottom wall");
+    std::vector<double> rho;
+    int rc = fetch_rho(c, rho);
+    if (rc) return rc;
+    const int DX = c->p.nx, DY = c->p.ny, DZ = c->p.nz, lo = c->p.nwall_lo, hi = c->p.nwall_hi;
+    const double Mid = (rho_g + rho_l) / 2, zw = lo - 0.5;
+    double area = 0, H = 0;
+    for (int i = 0; i < DX; ++i)
+        for (int j = 0; j < DY; ++j) {
+            const double *col = &rho[((int64_t)i * DY + j) * DZ];
+            if (col[lo] >= Mid) area += 1;
+            for (int k = lo; k < DZ - hi - 1; ++k)
+                if (col[k] >= Mid && col[k + 1] < Mid) {
+                    const double z = k + (col[k] - Mid) / (col[k] - col[k + 1]);
+                    if (z - zw > H) H = z - zw;
+                }
+        }
+    const double a = std::sqrt(area / 3.14159265358979323846);
+    double R = NAN, ang = NAN;
+    if (H > 0) {
+        R = (a * a + H * H) / (2 * H);
+        ang = std::atan2(a, R - H) * 180.0 / 3.14159265358979323846;
+    }
+    out[0] = ang; out[1] = H; out[2] = a; out[3] = R;
+    return CPMP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+"""Shared case definitions for the GPU parity tests: each builds the same
+lattice in the oracle and in the HIP library from one description."""
+import numpy as np
+
+import oracle_lib as orc
+
+RHO = {("PRW", 0.70): (0.0556211, 8.08045), ("PRW", 0.75): (0.108623, 7.67892),
+       ("CSE", 0.85): (0.0315818, 0.279262), ("RKE", 0.60): (0.0319804, 8.25467),
+       ("VDW", 0.80): (0.838834, 6.76447), ("RKS", 0.70): None, ("PRM", 0.70): None}
+
+
+def densities(eos, tr):
+    v = RHO.get((eos, tr))
+    if v is None:
+        A, B, Tc, Rc = orc.set_multiphase(eos, tr)
+        rc, g, l = orc.maxwell(eos, A, B, tr * Tc, 0.01 / B, 0.8 / B)
+        return g, l
+    return v
+
+
+def sphere_rho(nx, ny, nz, spheres, width, g, l, nwall=0, solid=0.0, noise=0.0, seed=0):
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    phi = np.zeros((nx, ny, nz))
+    for cx, cy, cz, R in spheres:
+        r = np.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2)
+        phi = np.maximum(phi, 0.5 * (1 - np.tanh(2 * (r - R) / width)))
+    rho = g + (l - g) * phi
+    if noise:
+        idx = np.arange(nx * ny * nz, dtype=np.uint64)
+        rho = rho.ravel() + noise * np.array([orc.noise(seed, int(i)) for i in idx])
+        rho = rho.reshape(nx, ny, nz)
+    if nwall:
+        rho[:, :, :nwall] = solid
+        rho[:, :, nz - nwall:] = solid
+    return rho.ravel()
+
+
+CASES = {
+    # reference geometry and init: PRW, Tr 0.7, walls 3+3, SetFlowField droplet
+    "ref_prw_walls": dict(dims=(32, 24, 24), eos="PRW", tr=0.70, nwall=3, init="reference",
+                          radius=8, width=5, mu_w=-0.02, steps=200),
+    # fully periodic Carnahan-Starling droplet with noise (the 64^3 configuration, shrunk)
+    "cse_periodic": dict(dims=(32, 32, 32), eos="CSE", tr=0.85, nwall=0, init="spheres",
+                         spheres=[(16, 16, 16, 8)], width=2.5, Ka=0.01, tau=1.0, noise=1e-6, steps=200),
+    # RK two touching droplets, periodic
+    "rke_coalesce": dict(dims=(40, 24, 24), eos="RKE", tr=0.60, nwall=0, init="spheres",
+                         spheres=[(12.5, 12, 12, 7), (27.5, 12, 12, 7)], width=5, noise=1e-4, steps=100),
+    # vdW with the reference's PR mu expression, 7-point Laplacian, simple wall, K1
+    "vdw_variants": dict(dims=(24, 20, 26), eos="VDW", tr=0.80, nwall=3, init="reference",
+                         radius=7, width=5, mu_form=1, lap_points=7, wall_scheme=0, K1=0.005,
+                         mu_w=0.01, steps=100),
+    # quasi-2-D domain as the shipped Common.h (DY = 1), wetting field varying in x
+    "prw_2d": dict(dims=(64, 1, 32), eos="PRW", tr=0.70, nwall=3, init="reference",
+                   radius=12, width=6, mu_w="ramp", steps=300),
+    # asymmetric wall layers and a top wall field of its own
+    "prm_walls_2_4": dict(dims=(20, 18, 30), eos="PRM", tr=0.70, nwall=2, nwall_hi=4, init="reference",
+                          radius=6, width=5, mu_w=-0.01, mu_w_hi=0.02, steps=100),
+}
+
+
+def build(case, cpmp, form="fp64"):
+    nx, ny, nz = case["dims"]
+    eos, tr = case["eos"], case["tr"]
+    kw = dict(eos=eos, tr=tr, nwall=case["nwall"], nwall_hi=case.get("nwall_hi"),
+              mu_form=case.get("mu_form", 0), lap_points=case.get("lap_points", 5),
+              wall_scheme=case.get("wall_scheme", 1), K1=case.get("K1", 0.0), K2=-case.get("K1", 0.0),
+              Ka=case.get("Ka", 0.001), tau=case.get("tau", 0.8))
+    po = orc.make_params(nx, ny, nz, **kw)
+    o = orc.Oracle(po)
+    pg = cpmp.make_params(nx, ny, nz, form=form, **kw)
+    if form == "fp16":
+        pg.rho0 = case.get("rho0", densities(eos, tr)[0])
+    g = cpmp.Lattice(pg)
+    mw = case.get("mu_w", 0.0)
+    if isinstance(mw, str):
+        mw = -0.03 + 0.05 * np.repeat(np.linspace(0, 1, nx), ny)
+    mwh = case.get("mu_w_hi", mw)
+    o.set_wall_mu(mw, mwh)
+    g.set_wall_mu(np.broadcast_to(np.asarray(mw, float), (nx * ny,)).copy(),
+                  np.broadcast_to(np.asarray(mwh, float), (nx * ny,)).copy())
+    dg, dl = densities(eos, tr)
+    if case["init"] == "reference":
+        o.init_reference(case["radius"], case["width"], dg, dl, 0.0)
+        g.init_reference(case["radius"], case["width"], dg, dl, 0.0)
+    else:
+        rho = sphere_rho(nx, ny, nz, case["spheres"], case["width"], dg, dl, case["nwall"],
+                         noise=case.get("noise", 0.0), seed=case.get("seed", 1))
+        o.init_from_rho(rho)
+        g.init_spheres(case["spheres"], case["width"], dg, dl, noise_amp=case.get("noise", 0.0),
+                       seed=case.get("seed", 1))
+    return o, g
+
+
+def rel_err(a, b, mask=None):
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))

@@ -1,0 +1,108 @@
+"""FP32 and FP16-storage forms against the FP64 form, conservation at the full
+256^3 size, checkpoints, Tecplot output, observables."""
+import os
+
+import numpy as np
+import pytest
+
+import gpu_cases as gc
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("form,tol", [("fp32", 2e-4), ("fp16", 2e-2)])
+def test_reduced_precision_forms_track_fp64(cpmp, form, tol):
+    case = gc.CASES["ref_prw_walls"]
+    _, g64 = gc.build(case, cpmp, "fp64")
+    _, gl = gc.build(case, cpmp, form)
+    g64.step(200)
+    gl.step(200)
+    r64, rl = g64.get("rho"), gl.get("rho")
+    fluid = g64.get("type") == 1
+    err = np.max(np.abs(rl[fluid] - r64[fluid])) / np.max(r64[fluid])
+    print(f"[forms] {form}: max |rho - rho_fp64| / max rho = {err:.3e}", flush=True)
+    assert err < tol
+    m64, ml = g64.reduce().mass, gl.reduce().mass
+    assert abs(ml - m64) / m64 < tol
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32", "fp16"])
+def test_mass_conservation_256cubed(cpmp, form):
+    """Full-size property: a periodic 256^3 RK coalescence run conserves mass
+    (FP64 sums on the device) and stays finite."""
+    n = 256
+    p = cpmp.make_params(n, n, n, eos="RKE", tr=0.6, nwall=0, form=form, rho0=0.0319804)
+    g = cpmp.Lattice(p)
+    g.init_spheres([(80, 128, 128, 48), (176, 128, 128, 48)], 5.0, 0.0319804, 8.25467, noise_amp=1e-4, seed=3)
+    m0 = g.reduce().mass
+    g.step(20)
+    s = g.reduce()
+    drift = abs(s.mass - m0) / m0
+    print(f"[256^3 {form}] mass drift after 20 steps {drift:.3e}, max|u| {s.max_speed:.3e}", flush=True)
+    assert s.bad_count == 0
+    assert drift < {"fp64": 1e-12, "fp32": 1e-5, "fp16": 1e-3}[form]
+    g.close()
+
+
+def test_checkpoint_roundtrip_and_corruption(cpmp, tmp_path):
+    case = gc.CASES["ref_prw_walls"]
+    _, g = gc.build(case, cpmp)
+    g.step(30)
+    path = str(tmp_path / "state.ckp")
+    g.checkpoint_save(path)
+    g.step(20)
+    ref = g.get("rho")
+    _, g2 = gc.build(case, cpmp)
+    g2.checkpoint_load(path)
+    assert g2.step_count == 30
+    g2.step(20)
+    assert np.array_equal(g2.get("rho"), ref)
+    bad = str(tmp_path / "bad.ckp")
+    with open(path, "rb") as fh:
+        data = bytearray(fh.read())
+    data[0] ^= 0xFF
+    with open(bad, "wb") as fh:
+        fh.write(data)
+    with pytest.raises(cpmp.CPMPError) as e:
+        g2.checkpoint_load(bad)
+    assert e.value.code == -4
+    with open(bad, "wb") as fh:
+        fh.write(bytes(data[:1000]))
+    with pytest.raises(cpmp.CPMPError):
+        g2.checkpoint_load(bad)
+
+
+def test_contact_angle_matches_oracle(cpmp):
+    case = gc.CASES["prw_2d"]
+    o, g = gc.build(case, cpmp)
+    o.step(300)
+    g.step(300)
+    dg, dl = gc.densities("PRW", 0.7)
+    a = g.contact_angle(0, dg, dl)
+    b = orc.contact_angle(o.p, o.get(orc.RHO), dg, dl, 0)
+    assert a["valid"] == b[7] == 1
+    got = np.array([a[k] for k in ("CaLeft", "CaRight", "CaCap", "XLeft", "XRight", "H", "L")])
+    assert np.allclose(got, b[:7], rtol=1e-8, atol=1e-8)
+
+
+def test_write_plt_layout(cpmp, tmp_path):
+    case = gc.CASES["vdw_variants"]
+    _, g = gc.build(case, cpmp)
+    g.step(5)
+    path = str(tmp_path / "FlowField_5.plt")
+    g.write_plt(path)
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"#!TDV112"
+    nx, ny, nz = case["dims"]
+    n = nx * ny * nz
+    rho = g.get("rho")
+    assert np.array_equal(np.frombuffer(raw[-8 * n:], np.float64), rho)
+    # reference writer, when built from the reference sources (oracle/_ref)
+    ref_bin = os.path.join(os.path.dirname(orc.LIB_PATH), "..", "_ref", "plt_ref")
+    if os.path.exists(ref_bin):
+        import subprocess
+        rho_path = str(tmp_path / "rho.bin")
+        rho.tofile(rho_path)
+        subprocess.check_call([ref_bin, rho_path, str(nx), str(ny), str(nz), "1", str(tmp_path), "FlowField_5"])
+        assert open(str(tmp_path / "FlowField_5.plt"), "rb").read() == raw

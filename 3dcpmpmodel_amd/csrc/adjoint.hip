@@ -1,9 +1,371 @@
 // adjoint.hip -- discrete adjoint of the step (no reference counterpart).
+//
+// Objective J = 1/2 sum_fluid (rho^N - target)^2 after N steps; gradients with
+// respect to the wall chemical-potential fields (bottom, top) and kappa (Ka).
+//
+// The state carried between steps is (f, rho) with u = j(f)/rho.  Reverse of one
+// step n, for adjoint state (fbar, rbar, ubar) of step n+1:
+//   moments : rho' = sum f', u' = j(f')/rho'   -> fhat = fbar + rbar + ubar.(e - u')/rho'
+//   stream  : push / half-way bounce-back       -> pull of fhat
+//   collide : m* = m - S(m - Meq(rho,u)) + S Qp + Meq(rho,u+F/rho) - Meq(rho,u), out = R m*
+//             -> fbar^n = M^T (I-S) R^T D, plus rho, u, F, grad(rho) cotangents
+//   force   : F = -rho G(mu) + G(rho)/3         -> the gradient stencils are antisymmetric,
+//             so their transposes are -G, applied in gather form
+//   mu      : mu = K^2 mu_eos(clamp rho) - Ka Lap(rho) (fluid), mu = mu_w (solid)
+//   wall    : solid rho = layered diagonal averages of the first fluid layer (symmetric)
+// The forward trajectory is recomputed segment by segment from population
+// checkpoints; within a segment the tape holds rho (after the wall fill) and u.
 #include "internal.h"
+#include "lbm_math.h"
+#include "device_common.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace cpmp {
+namespace {
+
+// sum_r w_r dMeq_r/drho and sum_r w_r dMeq_r/dv (rows 0..15 of FluidCuda.cu:268-286)
+__device__ __forceinline__ void meq_vjp(double rho, double vx, double vy, double vz, const double *w, double &drho,
+                                        double &dvx, double &dvy, double &dvz) {
+    const double usq = vx * vx + vy * vy + vz * vz, g9 = 2 * vx * vx - vy * vy - vz * vz, g11 = vy * vy - vz * vz;
+    const double t = 2.0 / 3.0;
+    drho = w[0] + w[1] * (-11 + 19 * usq) + w[2] * (3 - 5.5 * usq) + (w[3] - t * w[4]) * vx + (w[5] - t * w[6]) * vy
+           + (w[7] - t * w[8]) * vz + (w[9] - 0.5 * w[10]) * g9 + (w[11] - 0.5 * w[12]) * g11 + w[13] * vx * vy
+           + w[14] * vy * vz + w[15] * vx * vz;
+    dvx = rho * (38 * vx * w[1] - 11 * vx * w[2] + w[3] - t * w[4] + 4 * vx * w[9] - 2 * vx * w[10] + vy * w[13] + vz * w[15]);
+    dvy = rho * (38 * vy * w[1] - 11 * vy * w[2] + w[5] - t * w[6] - 2 * vy * w[9] + vy * w[10] + 2 * vy * w[11]
+                 - vy * w[12] + vx * w[13] + vz * w[14]);
+    dvz = rho * (38 * vz * w[1] - 11 * vz * w[2] + w[7] - t * w[8] - 2 * vz * w[9] + vz * w[10] - 2 * vz * w[11]
+                 + vz * w[12] + vy * w[14] + vx * w[15]);
+}
+
+// J and its seed: rb = rho - target on fluid nodes
+__global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const double *__restrict__ rho,
+                                               const double *__restrict__ target, double *__restrict__ rb,
+                                               double *__restrict__ part) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    double v = 0;
+    if (I < P.N) {
+        if (is_solid(P, (int)(I % P.nz))) {
+            rb[I] = 0;
+        } else {
+            const double d = rho[I] - target[I];
+            rb[I] = d;
+            v = 0.5 * d * d;
+        }
+    }
+    const double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// reverse of moments + streaming + collision at each fluid node
+__global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const double *__restrict__ fb_in,
+                                                     const double *__restrict__ rb_in, const double *__restrict__ vb_in,
+                                                     const double *__restrict__ rho1, const double *__restrict__ u1,
+                                                     const double *__restrict__ rho0, const double *__restrict__ u0,
+                                                     const double *__restrict__ mu0, double *__restrict__ fb_out,
+                                                     double *__restrict__ vb_out, double *__restrict__ rb_loc,
+                                                     double *__restrict__ Pf, double *__restrict__ Wf) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const long long N = P.N;
+    const Coord c = decompose(P, I);
+    if (is_solid(P, c.z)) {
+        rb_loc[I] = 0;
+        for (int a = 0; a < 3; ++a) { vb_out[a * N + I] = 0; Pf[a * N + I] = 0; Wf[a * N + I] = 0; }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fb_out[(long long)q * N + I] = 0;
+        return;
+    }
+    int xs[7], ys[7], zs[7];
+    wrap7(P, c, xs, ys, zs);
+    // pulled cotangents of the populations written by this node
+    double D[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int tz = zs[3 + cz(q)];
+        const bool bounce = is_solid(P, tz);
+        const long long y = bounce ? I : ((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz;
+        const int qq = bounce ? opp(q) : q;
+        const double ex = bounce ? -cx(q) : cx(q), ey = bounce ? -cy(q) : cy(q), ez = bounce ? -cz(q) : cz(q);
+        D[q] = fb_in[(long long)qq * N + y] + rb_in[y]
+               + (vb_in[y] * (ex - u1[y]) + vb_in[N + y] * (ey - u1[N + y]) + vb_in[2 * N + y] * (ez - u1[2 * N + y]))
+                     / rho1[y];
+    }
+    double m[Q], mb[Q], b[Q];
+    moments(D, m);
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+        mb[r] = m[r] * (1.0 / n2(r));
+        b[r] = (1.0 - P.ts[r]) * mb[r];
+    }
+    double fbar[Q];
+    inverse_scaled(b, fbar);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fb_out[(long long)q * N + I] = fbar[q];
+
+    const double rho = rho0[I], ux = u0[I], uy = u0[N + I], uz = u0[2 * N + I];
+    const Stencil<double, double> st{P, xs, ys, zs};
+    double gmx, gmy, gmz, grx, gry, grz;
+    st.grad(mu0, gmx, gmy, gmz);
+    st.grad(rho0, grx, gry, grz);
+    const double Fx = -gmx * rho + grx / 3, Fy = -gmy * rho + gry / 3, Fz = -gmz * rho + grz / 3;
+    const double ex = ux + Fx / rho, ey = uy + Fy / rho, ez = uz + Fz / rho;
+    double wu[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) wu[r] = (P.ts[r] - 1.0) * mb[r];
+    double dru, dux, duy, duz, dre, dex, dey, dez;
+    meq_vjp(rho, ux, uy, uz, wu, dru, dux, duy, duz);
+    meq_vjp(rho, ex, ey, ez, mb, dre, dex, dey, dez);
+    const double Fbx = dex / rho, Fby = dey / rho, Fbz = dez / rho;
+    double rbar = dru + dre - (dex * Fx + dey * Fy + dez * Fz) / (rho * rho);
+    rbar -= gmx * Fbx + gmy * Fby + gmz * Fbz;
+    vb_out[I] = dux + dex;
+    vb_out[N + I] = duy + dey;
+    vb_out[2 * N + I] = duz + dez;
+    double wx = Fbx / 3, wy = Fby / 3, wz = Fbz / 3;
+    if (P.qp_on) {
+        const double K1 = P.K1, c1 = -19 * (P.K1 + 3 * P.K2);
+        const double q1 = P.ts[1] * mb[1], q9 = P.ts[9] * mb[9], q11 = P.ts[11] * mb[11];
+        const double q13 = P.ts[13] * mb[13], q14 = P.ts[14] * mb[14], q15 = P.ts[15] * mb[15];
+        wx += q1 * 2 * c1 * grx - q9 * 4 * K1 * grx - q13 * K1 * gry - q15 * K1 * grz;
+        wy += q1 * 2 * c1 * gry + q9 * 2 * K1 * gry - q11 * 2 * K1 * gry - q13 * K1 * grx - q14 * K1 * grz;
+        wz += q1 * 2 * c1 * grz + q9 * 2 * K1 * grz + q11 * 2 * K1 * grz - q14 * K1 * gry - q15 * K1 * grx;
+    }
+    rb_loc[I] = rbar;
+    Pf[I] = rho * Fbx; Pf[N + I] = rho * Fby; Pf[2 * N + I] = rho * Fbz;
+    Wf[I] = wx; Wf[N + I] = wy; Wf[2 * N + I] = wz;
+}
+
+// transposed force stencils, all nodes: mub = sum_a G_a(P_a), rb_new = rb_loc - sum_a G_a(W_a)
+__global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const double *__restrict__ Pf,
+                                                     const double *__restrict__ Wf, const double *__restrict__ rb_loc,
+                                                     double *__restrict__ mub, double *__restrict__ rb_new) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I >= P.N) return;
+    const long long N = P.N;
+    const Coord c = decompose(P, I);
+    int xs[7], ys[7], zs[7];
+    wrap7(P, c, xs, ys, zs);
+    const Stencil<double, double> st{P, xs, ys, zs};
+    mub[I] = st.gx(Pf) + st.gy(Pf + N) + st.gz(Pf + 2 * N);
+    rb_new[I] = rb_loc[I] - (st.gx(Wf) + st.gy(Wf + N) + st.gz(Wf + 2 * N));
+}
+
+// transposed chemical potential: Laplacian of the fluid mu cotangent, EOS slope,
+// and the kappa gradient (per-block partial sums)
+__global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const double *__restrict__ rho0,
+                                                const double *__restrict__ mub, double *__restrict__ rb_new,
+                                                double *__restrict__ kapart) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    double kap = 0;
+    if (I < P.N) {
+        const Coord c = decompose(P, I);
+        int xs[7], ys[7], zs[7];
+        wrap7(P, c, xs, ys, zs);
+        auto idx = [&](int dx, int dy, int dz) { return ((long long)xs[dx + 3] * P.ny + ys[dy + 3]) * P.nz + zs[dz + 3]; };
+        auto mm = [&](int dx, int dy, int dz) { return is_solid(P, zs[dz + 3]) ? 0.0 : mub[idx(dx, dy, dz)]; };
+        auto rr = [&](int dx, int dy, int dz) { return rho0[idx(dx, dy, dz)]; };
+        auto lap = [&](auto &&v) {
+            if (P.lap_points == 7) {
+                auto ax = [&](int ux, int uy, int uz) {
+                    return 2 * v(-3 * ux, -3 * uy, -3 * uz) - 27 * v(-2 * ux, -2 * uy, -2 * uz) + 270 * v(-ux, -uy, -uz)
+                           + 270 * v(ux, uy, uz) - 27 * v(2 * ux, 2 * uy, 2 * uz) + 2 * v(3 * ux, 3 * uy, 3 * uz);
+                };
+                return (-v(0, 0, 0) * 490 * 3 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 180;
+            }
+            auto ax = [&](int ux, int uy, int uz) {
+                return -v(-2 * ux, -2 * uy, -2 * uz) + v(-ux, -uy, -uz) * 16 + v(ux, uy, uz) * 16 - v(2 * ux, 2 * uy, 2 * uz);
+            };
+            return (-v(0, 0, 0) * 90 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 12;
+        };
+        double acc = -P.Ka * lap(mm);
+        if (!is_solid(P, c.z)) {
+            const double r = rho0[I], mu_b = mub[I];
+            acc += P.K2s * dmu_eos(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho(r)) * clamp_drho(r) * mu_b;
+            kap = -mu_b * lap(rr);
+        }
+        rb_new[I] += acc;
+    }
+    const double s = block_sum(kap);
+    if (threadIdx.x == 0) kapart[blockIdx.x] = s;
+}
+
+// wall-field gradients: solid mu = mu_w of the column
+__global__ void __launch_bounds__(TPB) k_adj_wall_g(const DevParams P, const double *__restrict__ mub,
+                                                    double *__restrict__ glo, double *__restrict__ ghi) {
+    const long long col = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (col >= (long long)P.nx * P.ny) return;
+    const double *m = mub + col * P.nz;
+    double a = 0, b = 0;
+    for (int k = 0; k < P.lo; ++k) a += m[k];
+    for (int k = P.nz - P.hi; k < P.nz; ++k) b += m[k];
+    glo[col] += a;
+    ghi[col] += b;
+}
+
+// transposed wall fill, one layer pass (run deepest pass first)
+__global__ void __launch_bounds__(TPB) k_adj_wall_pass(const DevParams P, double *__restrict__ rb, int pass) {
+    const long long col = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (col >= (long long)P.nx * P.ny) return;
+    const int x = (int)(col / P.ny), y = (int)(col % P.ny);
+    const int x1 = wrapc(x - 1, P.nx), x2 = wrapc(x + 1, P.nx), y1 = wrapc(y - 1, P.ny), y2 = wrapc(y + 1, P.ny);
+    auto at = [&](int xx, int yy, int k) { return ((long long)xx * P.ny + yy) * P.nz + k; };
+    auto avg = [&](int k) {
+        return (rb[at(x, y, k)] * 2 + rb[at(x1, y1, k)] + rb[at(x1, y2, k)] + rb[at(x2, y1, k)] + rb[at(x2, y2, k)]) / 6;
+    };
+    if (pass < P.lo) {
+        const int kd = P.lo - 1 - pass, ks = P.wall_scheme ? kd + 1 : P.lo;
+        rb[at(x, y, ks)] += avg(kd);
+    }
+    if (pass < P.hi) {
+        const int kd = P.nz - P.hi + pass, ks = P.wall_scheme ? kd - 1 : P.nz - 1 - P.hi;
+        rb[at(x, y, ks)] += avg(kd);
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_adj_zero_solid(const DevParams P, double *__restrict__ rb) {
+    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (I < P.N && is_solid(P, (int)(I % P.nz))) rb[I] = 0;
+}
+
+// acc[0] += sum(part[0..nb)) in a fixed order
+__global__ void __launch_bounds__(TPB) k_sum_into(const double *__restrict__ part, long long nb, double *__restrict__ acc) {
+    double s = 0;
+    for (long long b = threadIdx.x; b < nb; b += TPB) s += part[b];
+    s = block_sum(s);
+    if (threadIdx.x == 0) acc[0] += s;
+}
+
+}  // namespace
+}  // namespace cpmp
 
 using namespace cpmp;
 
+#define AHIP(expr)                                                                                \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) { rc = set_error(CPMP_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); goto done; } \
+    } while (0)
+
 extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, double *g_lo, double *g_hi, double *g_ka) {
-    (void)c; (void)d; (void)J; (void)g_lo; (void)g_hi; (void)g_ka;
-    return set_error(CPMP_EINVAL, "adjoint not available in this build");
+    if (!c || !c->stream || !d || !d->target || !J) return set_error(CPMP_EINVAL, "bad arguments");
+    if (c->form != CPMP_FORM_FP64) return set_error(CPMP_EINVAL, "the adjoint runs in the FP64 form only");
+    if (d->nsteps < 1 || d->checkpoint_every < 1) return set_error(CPMP_EINVAL, "nsteps and checkpoint_every must be >= 1");
+    int rc = CPMP_OK;
+    const long long N = c->N, NXY = c->NXY;
+    const int C = std::min(d->checkpoint_every, d->nsteps);
+    const int nseg = (d->nsteps + C - 1) / C;
+    const unsigned nb = blocks_for(N), nbc = blocks_for(NXY);
+    const hipStream_t s = c->stream;
+    const DevParams &P = c->dp;
+    const int passes = std::max(P.lo, P.hi);
+    const int64_t step0 = c->step;
+    double *ckpt = nullptr, *tape = nullptr, *buf = nullptr, *tgt = nullptr;
+    double *fbA, *fbB, *rb, *rbn, *rbl, *vbA, *vbB, *Pf, *Wf, *mub, *part, *gl, *gh, *acc, *fin_f, *fin_r;
+    const size_t per_ck = 20 * (size_t)N;          // f (19N) + rho (N)
+    const size_t per_tape = 4 * (size_t)N;          // rho (N) + u (3N)
+    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 * 2 + 3 + 3 + 1) + nb + 2 * NXY + 4 + 20 * (size_t)N;
+    double acc_h[2] = {0, 0};
+
+    AHIP(hipSetDevice(c->p.device));
+    AHIP(hipMalloc((void **)&ckpt, nseg * per_ck * sizeof(double)));
+    AHIP(hipMalloc((void **)&tape, (C + 1) * per_tape * sizeof(double)));
+    AHIP(hipMalloc((void **)&buf, nbuf * sizeof(double)));
+    AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
+    {
+        double *p = buf;
+        fbA = p; p += 19 * N; fbB = p; p += 19 * N;
+        rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
+        vbA = p; p += 3 * N; vbB = p; p += 3 * N;
+        Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
+        part = p; p += nb; gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
+        fin_f = p; p += 19 * N; fin_r = p; p += N;
+    }
+    AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
+    AHIP(hipMemsetAsync(buf, 0, nbuf * sizeof(double), s));
+
+    // ---- forward with checkpoints ----
+    for (int sg = 0; sg < nseg; ++sg) {
+        double *ck = ckpt + sg * per_ck;
+        AHIP(hipMemcpyAsync(ck, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+        AHIP(hipMemcpyAsync(ck + 19 * N, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+        const int L = std::min(C, d->nsteps - sg * C);
+        for (int k = 0; k < L; ++k) AHIP(launch_step(c, false));
+    }
+    AHIP(hipMemcpyAsync(fin_f, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    AHIP(hipMemcpyAsync(fin_r, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_adj_J, dim3(nb), dim3(TPB), 0, s, P, (const double *)c->rho, (const double *)tgt, rb, part);
+    AHIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nb, acc);
+    AHIP(hipGetLastError());
+
+    // ---- reverse, segment by segment ----
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const double *ck = ckpt + sg * per_ck;
+        AHIP(hipMemcpyAsync(c->f[c->cur], ck, 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+        AHIP(hipMemcpyAsync(c->rho, ck + 19 * N, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+        const int L = std::min(C, d->nsteps - sg * C);
+        for (int k = 0; k < L; ++k) {
+            double *tr = tape + k * per_tape;
+            AHIP(launch_wall(c));
+            AHIP(hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            AHIP(launch_mu(c));
+            AHIP(launch_collide(c, false, tr + N));
+            AHIP(launch_macro(c));
+        }
+        {
+            double *tr = tape + L * per_tape;
+            AHIP(hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            AHIP(launch_velocity(c, tr + N));
+        }
+        for (int k = L - 1; k >= 0; --k) {
+            const double *t0 = tape + k * per_tape, *t1 = tape + (k + 1) * per_tape;
+            // mu^n of the step (solid mu = wall field) from the taped density
+            AHIP(hipMemcpyAsync(c->rho, t0, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            AHIP(launch_wall(c));
+            AHIP(launch_mu(c));
+            hipLaunchKernelGGL(k_adj_collide, dim3(nb), dim3(TPB), 0, s, P, (const double *)fbA, (const double *)rb,
+                               (const double *)vbA, t1, t1 + N, t0, t0 + N, (const double *)c->mu, fbB, vbB, rbl, Pf, Wf);
+            AHIP(hipGetLastError());
+            hipLaunchKernelGGL(k_adj_stencil, dim3(nb), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
+                               (const double *)rbl, mub, rbn);
+            AHIP(hipGetLastError());
+            hipLaunchKernelGGL(k_adj_mu, dim3(nb), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, part);
+            AHIP(hipGetLastError());
+            hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nb, acc + 1);
+            AHIP(hipGetLastError());
+            if (passes > 0) {
+                hipLaunchKernelGGL(k_adj_wall_g, dim3(nbc), dim3(TPB), 0, s, P, (const double *)mub, gl, gh);
+                AHIP(hipGetLastError());
+                for (int pass = passes - 1; pass >= 0; --pass) {
+                    hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass);
+                    AHIP(hipGetLastError());
+                }
+                hipLaunchKernelGGL(k_adj_zero_solid, dim3(nb), dim3(TPB), 0, s, P, rbn);
+                AHIP(hipGetLastError());
+            }
+            std::swap(fbA, fbB);
+            std::swap(vbA, vbB);
+            std::swap(rb, rbn);
+        }
+    }
+    AHIP(hipMemcpyAsync(acc_h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (g_lo) AHIP(hipMemcpyAsync(g_lo, gl, NXY * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (g_hi) AHIP(hipMemcpyAsync(g_hi, gh, NXY * sizeof(double), hipMemcpyDeviceToHost, s));
+    // leave the context at the end of the forward run
+    AHIP(hipMemcpyAsync(c->f[c->cur], fin_f, 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    AHIP(hipMemcpyAsync(c->rho, fin_r, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    AHIP(hipStreamSynchronize(s));
+    *J = acc_h[0];
+    if (g_ka) *g_ka = acc_h[1];
+    c->step = step0 + d->nsteps;
+    c->mv_valid = false;
+done:
+    if (ckpt) hipFree(ckpt);
+    if (tape) hipFree(tape);
+    if (buf) hipFree(buf);
+    if (tgt) hipFree(tgt);
+    return rc;
 }

@@ -62,7 +62,7 @@ int set_error(int code, const char *fmt, ...);
 // launchers (kernels.hip); all enqueue on ctx->stream
 hipError_t launch_wall(cpmp_ctx *c);
 hipError_t launch_mu(cpmp_ctx *c);
-hipError_t launch_collide(cpmp_ctx *c, bool want_mv);
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout = nullptr);
 hipError_t launch_macro(cpmp_ctx *c);
 hipError_t launch_step(cpmp_ctx *c, bool want_mv);
 hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, double rg, double rl, double solid);

@@ -8,8 +8,11 @@
 // (f stored as f - w_q*rho0 in _Float16; rho, mu float).
 #include "internal.h"
 #include "lbm_math.h"
+#include "device_common.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 namespace cpmp {
@@ -45,20 +48,6 @@ __device__ __forceinline__ TF pop_val(TC v, int q, double rho0) {
     else return (TF)v;
 }
 
-__device__ __forceinline__ bool is_solid(const DevParams &P, int z) { return z < P.lo || z >= P.nz - P.hi; }
-__device__ __forceinline__ int wrapc(int a, int n) { a %= n; return a < 0 ? a + n : a; }
-
-struct Coord { int x, y, z; };
-__device__ __forceinline__ Coord decompose(const DevParams &P, long long I) {
-    Coord c;
-    c.z = (int)(I % P.nz);
-    const long long r = I / P.nz;
-    c.y = (int)(r % P.ny);
-    c.x = (int)(r / P.ny);
-    return c;
-}
-
-constexpr int TPB = 256;
 
 // ---------------------------------------------------------------------------
 // Wall fill, one pass (ChemBoundary, FluidCuda.cu:376-401, as ordered layers).
@@ -102,8 +91,10 @@ __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restr
     if (I >= P.N) return;
     const Coord c = decompose(P, I);
     if (is_solid(P, c.z)) return;
+    int xs[7], ys[7], zs[7];
+    wrap7(P, c, xs, ys, zs);
     auto at = [&](int dx, int dy, int dz) {
-        return (double)rho[((long long)wrapc(c.x + dx, P.nx) * P.ny + wrapc(c.y + dy, P.ny)) * P.nz + wrapc(c.z + dz, P.nz)];
+        return (double)rho[((long long)xs[dx + 3] * P.ny + ys[dy + 3]) * P.nz + zs[dz + 3]];
     };
     const double d = (double)rho[I];
     const double m = mu_eos(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho(d));
@@ -123,28 +114,6 @@ __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restr
     mu[I] = (TS)(P.K2s * m - kal);
 }
 
-// ---------------------------------------------------------------------------
-// Isotropic 6th-order gradient of Common.h:222-241 at (x,y,z) with wrapped
-// coordinates xs/ys/zs[s+3] = coordinate + s.
-template <class TS, class TC>
-struct Stencil {
-    const DevParams &P;
-    const int *xs, *ys, *zs;
-    __device__ __forceinline__ TC at(const TS *a, int dx, int dy, int dz) const {
-        return (TC)a[((long long)xs[dx + 3] * P.ny + ys[dy + 3]) * P.nz + zs[dz + 3]];
-    }
-    __device__ __forceinline__ TC dd(const TS *a, int ux, int uy, int uz) const {
-        return -at(a, -3 * ux, -3 * uy, -3 * uz) + TC(9) * at(a, -2 * ux, -2 * uy, -2 * uz)
-               - TC(45) * at(a, -ux, -uy, -uz) + TC(45) * at(a, ux, uy, uz)
-               - TC(9) * at(a, 2 * ux, 2 * uy, 2 * uz) + at(a, 3 * ux, 3 * uy, 3 * uz);
-    }
-    __device__ __forceinline__ void grad(const TS *a, TC &gx, TC &gy, TC &gz) const {
-        const TC s = TC(1.0 / 360.0);
-        gx = (TC(2) * dd(a, 1, 0, 0) + dd(a, 1, 0, 1) + dd(a, 1, 0, -1) + dd(a, 1, 1, 0) + dd(a, 1, -1, 0)) * s;
-        gy = (TC(2) * dd(a, 0, 1, 0) + dd(a, 0, 1, 1) + dd(a, 0, 1, -1) + dd(a, 1, 1, 0) + dd(a, -1, 1, 0)) * s;
-        gz = (TC(2) * dd(a, 0, 0, 1) + dd(a, 1, 0, 1) + dd(a, -1, 0, 1) + dd(a, 0, 1, 1) + dd(a, 0, -1, 1)) * s;
-    }
-};
 
 // ---------------------------------------------------------------------------
 // Fused non-ideal force + MRT collision + exact-difference forcing + push
@@ -153,7 +122,7 @@ struct Stencil {
 template <class TF, class TS, class TC, bool MV>
 __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__restrict__ fin, TF *__restrict__ fout,
                                                  const TS *__restrict__ rho, const TS *__restrict__ mu,
-                                                 double *__restrict__ mv) {
+                                                 double *__restrict__ mv, double *__restrict__ uout) {
     const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
     if (I >= P.N) return;
     const Coord c = decompose(P, I);
@@ -177,6 +146,11 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
     TC jx, jy, jz;
     momentum(f, jx, jy, jz);
     const TC ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    if (uout) {   // velocity tape for the adjoint's recomputed forward
+        uout[I] = (double)ux;
+        uout[N + I] = (double)uy;
+        uout[2 * N + I] = (double)uz;
+    }
 
     TC gmx, gmy, gmz, grx, gry, grz;
     st.grad(mu, gmx, gmy, gmz);
@@ -209,6 +183,134 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
         } else {
             const long long T = ((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz;
             fout[(long long)q * N + T] = pop_val<TF, TC>(out[q], q, P.rho0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same fused step as k_collide, organised for gfx950: a workgroup owns an
+// in-plane tile of MTY x MTZ nodes (z contiguous, 32 lanes per row) and marches
+// along x over MXC planes.  rho and mu of planes x-3..x+3 with an in-plane halo
+// of 3 sit in an LDS ring, so each value is fetched from L2/HBM about once per
+// tile instead of 54 times per stencil; the populations of plane x are loaded
+// (coalesced) before the ring update so their latency overlaps the barrier.
+constexpr int MTY = 8, MTZ = 32, MH = 3, MXC = 32;
+constexpr int MLY = MTY + 2 * MH, MLZ = MTZ + 2 * MH, MLP = MLY * MLZ;
+
+template <class TS, class TC>
+struct LdsStencil {
+    const TS (*a)[MLP];
+    const int *slot;   // ring slot of plane x + dx, dx = -3..3
+    int base;          // (ty + MH) * MLZ + tz + MH
+    __device__ __forceinline__ TC at(int dx, int dy, int dz) const { return (TC)a[slot[dx + 3]][base + dy * MLZ + dz]; }
+    __device__ __forceinline__ TC dd(int ux, int uy, int uz) const {
+        return -at(-3 * ux, -3 * uy, -3 * uz) + TC(9) * at(-2 * ux, -2 * uy, -2 * uz) - TC(45) * at(-ux, -uy, -uz)
+               + TC(45) * at(ux, uy, uz) - TC(9) * at(2 * ux, 2 * uy, 2 * uz) + at(3 * ux, 3 * uy, 3 * uz);
+    }
+    __device__ __forceinline__ void grad(TC &gx, TC &gy, TC &gz) const {
+        const TC s = TC(1.0 / 360.0);
+        gx = (TC(2) * dd(1, 0, 0) + dd(1, 0, 1) + dd(1, 0, -1) + dd(1, 1, 0) + dd(1, -1, 0)) * s;
+        gy = (TC(2) * dd(0, 1, 0) + dd(0, 1, 1) + dd(0, 1, -1) + dd(1, 1, 0) + dd(-1, 1, 0)) * s;
+        gz = (TC(2) * dd(0, 0, 1) + dd(1, 0, 1) + dd(-1, 0, 1) + dd(0, 1, 1) + dd(0, -1, 1)) * s;
+    }
+};
+
+template <class TF, class TS, class TC, bool MV>
+__global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const TF *__restrict__ fin,
+                                                       TF *__restrict__ fout, const TS *__restrict__ rho,
+                                                       const TS *__restrict__ mu, double *__restrict__ mv,
+                                                       double *__restrict__ uout, int ntz, int nty) {
+    __shared__ TS s_rho[7][MLP];
+    __shared__ TS s_mu[7][MLP];
+    const long long N = P.N;
+    // XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs,
+    // so give each XCD a contiguous range of tiles (shared halos stay in its L2).
+    int t = blockIdx.x;
+    if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
+    const int tzi = t % ntz;
+    t /= ntz;
+    const int tyi = t % nty;
+    const int ch = t / nty;
+    const int y0 = tyi * MTY, z0 = tzi * MTZ;
+    const int xa = ch * MXC, xb = min(xa + MXC, P.nx);
+    const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
+    const int y = y0 + ty, z = z0 + tz;
+    const bool active = (y < P.ny) && (z < P.nz) && !is_solid(P, z);
+
+    auto load_plane = [&](int xg, int slot) {
+        const int xw = wrapc(xg, P.nx);
+        for (int e = threadIdx.x; e < MLP; e += TPB) {
+            const int ly = e / MLZ, lz = e - ly * MLZ;
+            const long long gi = ((long long)xw * P.ny + wrapc(y0 - MH + ly, P.ny)) * P.nz + wrapc(z0 - MH + lz, P.nz);
+            s_rho[slot][e] = rho[gi];
+            s_mu[slot][e] = mu[gi];
+        }
+    };
+    for (int k = -MH; k < MH; ++k) load_plane(xa + k, k + MH);
+
+    const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
+    const int yp = wrap_off(ym, 1, P.ny), yn = wrap_off(ym, -1, P.ny);
+    const int zp = wrap_off(zm, 1, P.nz), zn = wrap_off(zm, -1, P.nz);
+    const int base = (ty + MH) * MLZ + tz + MH;
+    TC ts[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ts[q] = (TC)P.ts[q];
+
+    for (int x = xa; x < xb; ++x) {
+        const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
+        TC f[Q];
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, (long long)q * N + I, q, P.rho0);
+        }
+        __syncthreads();                       // every thread is done with plane x-4
+        load_plane(x + MH, (x - xa + 2 * MH) % 7);
+        __syncthreads();
+        if (active) {
+            int slot[7];
+#pragma unroll
+            for (int d = 0; d < 7; ++d) slot[d] = (x - xa + d) % 7;
+            const LdsStencil<TS, TC> sm{s_mu, slot, base}, sr{s_rho, slot, base};
+            const TC r = sr.at(0, 0, 0);
+            const TC inv = TC(1) / r;
+            TC jx, jy, jz;
+            momentum(f, jx, jy, jz);
+            const TC ux = jx * inv, uy = jy * inv, uz = jz * inv;
+            if (uout) {
+                uout[I] = (double)ux;
+                uout[N + I] = (double)uy;
+                uout[2 * N + I] = (double)uz;
+            }
+            TC gmx, gmy, gmz, grx, gry, grz;
+            sm.grad(gmx, gmy, gmz);
+            sr.grad(grx, gry, grz);
+            const TC third = TC(1.0 / 3.0);
+            const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
+            if (MV) {
+                mv[I] = (double)(ux + Fx * inv * TC(0.5));
+                mv[N + I] = (double)(uy + Fy * inv * TC(0.5));
+                mv[2 * N + I] = (double)(uz + Fz * inv * TC(0.5));
+            }
+            TC out[Q];
+            if (P.qp_on) {
+                TC qp[Q];
+                qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
+                collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, qp, out);
+            } else {
+                collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, (const TC *)nullptr, out);
+            }
+            const int xp = wrap_off(x, 1, P.nx), xn = wrap_off(x, -1, P.nx);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int tzq = cz(q) > 0 ? zp : (cz(q) < 0 ? zn : zm);
+                if (is_solid(P, tzq)) {
+                    fout[(long long)opp(q) * N + I] = pop_val<TF, TC>(out[q], opp(q), P.rho0);
+                } else {
+                    const int txq = cx(q) > 0 ? xp : (cx(q) < 0 ? xn : x);
+                    const int tyq = cy(q) > 0 ? yp : (cy(q) < 0 ? yn : ym);
+                    fout[(long long)q * N + ((long long)txq * P.ny + tyq) * P.nz + tzq] = pop_val<TF, TC>(out[q], q, P.rho0);
+                }
+            }
         }
     }
 }
@@ -357,21 +459,6 @@ __global__ void __launch_bounds__(TPB) k_type(const DevParams P, double *__restr
 constexpr int RED_BLOCKS = 512;
 constexpr int RED_VALS = 12;   // mass, bad, first_bad, rmin, rmax, maxspeed2, mom x/y/z, kin, count, -
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
 // kinds: 0 sum, 1 min, 2 max
 __device__ __forceinline__ void block_reduce(double (&v)[RED_VALS], const int (&kind)[RED_VALS], double *out) {
     __shared__ double sh[TPB / 64][RED_VALS];
@@ -430,7 +517,6 @@ __global__ void __launch_bounds__(TPB) k_reduce_final(const double *__restrict__
     block_reduce(v, kind, out);
 }
 
-inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
 }  // namespace
 
@@ -457,19 +543,39 @@ hipError_t launch_mu(cpmp_ctx *c) {
     });
 }
 
-hipError_t launch_collide(cpmp_ctx *c, bool want_mv) {
+// CPMP_COLLIDE=v1 selects the one-thread-per-node kernel (stencils from cache)
+static int collide_variant() {
+    static const int v = [] {
+        const char *e = std::getenv("CPMP_COLLIDE");
+        return (e && std::strcmp(e, "v1") == 0) ? 1 : 2;
+    }();
+    return v;
+}
+
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;
         using TC = typename decltype(fm)::TC;
         const TF *fin = (const TF *)c->f[c->cur];
         TF *fout = (TF *)c->f[c->cur ^ 1];
-        if (want_mv)
-            hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, fin,
-                               fout, (const TS *)c->rho, (const TS *)c->mu, c->mv);
-        else
-            hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, fin,
-                               fout, (const TS *)c->rho, (const TS *)c->mu, c->mv);
+        if (collide_variant() == 1) {
+            if (want_mv)
+                hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                                   fin, fout, (const TS *)c->rho, (const TS *)c->mu, c->mv, uout);
+            else
+                hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                                   fin, fout, (const TS *)c->rho, (const TS *)c->mu, c->mv, uout);
+        } else {
+            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + MXC - 1) / MXC;
+            const dim3 grid((unsigned)(ntz * nty * nch));
+            if (want_mv)
+                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true>), grid, dim3(TPB), 0, c->stream, c->dp, fin, fout,
+                                   (const TS *)c->rho, (const TS *)c->mu, c->mv, uout, ntz, nty);
+            else
+                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false>), grid, dim3(TPB), 0, c->stream, c->dp, fin, fout,
+                                   (const TS *)c->rho, (const TS *)c->mu, c->mv, uout, ntz, nty);
+        }
         c->cur ^= 1;
         return hipGetLastError();
     });

@@ -1,0 +1,179 @@
+"""HIP discrete adjoint: against the oracle's reverse mode (FP64 reference
+gradient), central finite differences of the HIP forward (h = 1e-6, FP64), and
+checkpoint-interval invariance; then the 96x96x64 adjoint configuration."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def fourier_field(nx, ny, seed, modes=4, amp=0.2, mean=0.0):
+    """smooth random wall field: sum of `modes` low Fourier modes, max |.| = amp"""
+    rng = np.random.default_rng(seed)
+    x, y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    f = np.zeros((nx, ny))
+    for _ in range(modes):
+        k, l = rng.integers(0, 3, 2)
+        if k == 0 and l == 0:
+            k = 1
+        f += rng.standard_normal() * np.cos(2 * np.pi * (k * x / nx + l * y / ny) + rng.uniform(0, 2 * np.pi))
+    return (mean + amp * f / np.max(np.abs(f))).ravel()
+
+
+def small_case(cpmp, nx=16, ny=12, nz=20, K1=0.0):
+    g, l = 0.108623, 7.67892     # PRW Tr = 0.75 (CoCurve_PRW_2.txt)
+    kw = dict(eos="PRW", tr=0.75, nwall=3, Ka=0.002, K1=K1, K2=-K1)
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    r = np.sqrt((x - nx / 2) ** 2 + (y - ny / 2) ** 2 + (z - 2.5) ** 2)
+    rho0 = g + (l - g) * 0.5 * (1 - np.tanh(2 * (r - 6) / 4.0))
+    rho0[:, :, :3] = 0
+    rho0[:, :, nz - 3:] = 0
+    mu_lo = fourier_field(nx, ny, 4, amp=0.02, mean=0.0)
+    mu_hi = np.full(nx * ny, 0.01)
+    rng = np.random.default_rng(9)
+    target = rho0.ravel() * (1 + 0.02 * rng.standard_normal(rho0.size))
+    return kw, rho0.ravel(), mu_lo, mu_hi, target
+
+
+def gpu_J(cpmp, kw, dims, rho0, mu_lo, mu_hi, target, nsteps, Ka=None):
+    k = dict(kw)
+    if Ka is not None:
+        k["Ka"] = Ka
+    g = cpmp.Lattice(cpmp.make_params(*dims, **k))
+    g.set_wall_mu(mu_lo, mu_hi)
+    g.init_from_rho(rho0)
+    g.step(nsteps)
+    rho = g.get("rho")
+    fl = g.get("type") == 1
+    g.close()
+    return 0.5 * np.sum((rho[fl] - target[fl]) ** 2)
+
+
+@pytest.mark.parametrize("K1", [0.0, 0.01])
+def test_adjoint_matches_oracle_reverse_mode(cpmp, K1):
+    dims = (16, 12, 20)
+    kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims, K1=K1)
+    nsteps = 30
+    g = cpmp.Lattice(cpmp.make_params(*dims, **kw))
+    g.set_wall_mu(mu_lo, mu_hi)
+    g.init_from_rho(rho0)
+    J, glo, ghi, gka = g.adjoint(nsteps, target, checkpoint_every=7)
+    o = orc.Oracle(orc.make_params(*dims, **kw))
+    o.set_wall_mu(mu_lo, mu_hi)
+    o.init_from_rho(rho0)
+    Jo, glo_o, ghi_o, gka_o = o.adjoint(nsteps, target)
+    print(f"[adjoint] K1={K1}: J {J:.12e} vs {Jo:.12e}; |g_lo| {np.linalg.norm(glo):.6e}", flush=True)
+    assert J == pytest.approx(Jo, rel=1e-10)
+    assert np.max(np.abs(glo - glo_o)) < 1e-7 * np.max(np.abs(glo_o))
+    assert np.max(np.abs(ghi - ghi_o)) < 1e-7 * np.max(np.abs(ghi_o)) + 1e-12
+    assert gka == pytest.approx(gka_o, rel=1e-7)
+    # the state is left at the end of the forward run
+    assert g.step_count == nsteps
+    assert np.allclose(g.get("rho"), o.get(orc.RHO), rtol=1e-10)
+
+
+def test_adjoint_checkpoint_interval_invariance(cpmp):
+    dims = (16, 12, 20)
+    kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims)
+    res = []
+    for ck in (1, 5, 40):
+        g = cpmp.Lattice(cpmp.make_params(*dims, **kw))
+        g.set_wall_mu(mu_lo, mu_hi)
+        g.init_from_rho(rho0)
+        res.append(g.adjoint(40, target, checkpoint_every=ck))
+        g.close()
+    for r in res[1:]:
+        assert r[0] == pytest.approx(res[0][0], rel=1e-14)
+        assert np.allclose(r[1], res[0][1], rtol=1e-12, atol=1e-14 * np.max(np.abs(res[0][1])))
+        assert r[3] == pytest.approx(res[0][3], rel=1e-12)
+
+
+def test_adjoint_matches_central_differences(cpmp):
+    dims = (16, 12, 20)
+    kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims)
+    nsteps = 60
+    g = cpmp.Lattice(cpmp.make_params(*dims, **kw))
+    g.set_wall_mu(mu_lo, mu_hi)
+    g.init_from_rho(rho0)
+    J, glo, ghi, gka = g.adjoint(nsteps, target, checkpoint_every=20)
+    rng = np.random.default_rng(6)
+    h = 1e-6
+    for i in range(8):
+        d = rng.standard_normal(mu_lo.size)
+        fd = (gpu_J(cpmp, kw, dims, rho0, mu_lo + h * d, mu_hi, target, nsteps)
+              - gpu_J(cpmp, kw, dims, rho0, mu_lo - h * d, mu_hi, target, nsteps)) / (2 * h)
+        ad = glo @ d
+        print(f"[adjoint-fd] dir {i}: adjoint {ad:.10e} fd {fd:.10e}", flush=True)
+        assert ad == pytest.approx(fd, rel=1e-4)
+    hk = 1e-7
+    fd = (gpu_J(cpmp, kw, dims, rho0, mu_lo, mu_hi, target, nsteps, Ka=kw["Ka"] + hk)
+          - gpu_J(cpmp, kw, dims, rho0, mu_lo, mu_hi, target, nsteps, Ka=kw["Ka"] - hk)) / (2 * hk)
+    assert gka == pytest.approx(fd, rel=1e-4)
+
+
+def c4_setup(cpmp, nsteps_target):
+    """96x96x64 sessile PR droplet at Tr = 0.75; wall field seed 4, target from a
+    forward run with the wall field of seed 5."""
+    nx, ny, nz = 96, 96, 64
+    kw = dict(eos="PRW", tr=0.75, nwall=3)
+    g, l = 0.108623, 7.67892
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    r = np.sqrt((x - nx / 2) ** 2 + (y - ny / 2) ** 2 + (z - 2.5) ** 2)
+    rho0 = g + (l - g) * 0.5 * (1 - np.tanh(2 * (r - 24) / 5.0))
+    rho0[:, :, :3] = 0
+    rho0[:, :, nz - 3:] = 0
+    rho0 = rho0.ravel()
+    mu_design = fourier_field(nx, ny, 4, amp=0.2)
+    mu_target = fourier_field(nx, ny, 5, amp=0.2)
+    lat = cpmp.Lattice(cpmp.make_params(nx, ny, nz, **kw))
+    lat.set_wall_mu(mu_target, 0.0)
+    lat.init_from_rho(rho0)
+    lat.step(nsteps_target)
+    target = lat.get("rho")
+    lat.close()
+    return (nx, ny, nz), kw, rho0, mu_design, target
+
+
+def test_adjoint_96x96x64_configuration(cpmp):
+    """2000 steps, checkpoints every 100, FD (h = 1e-6, FP64) on 16 random
+    directions of the wall field; adjoint/forward time ratio reported."""
+    nsteps = 2000
+    dims, kw, rho0, mu, target = c4_setup(cpmp, nsteps)
+    hi = np.zeros_like(mu)
+
+    def run(m):
+        lat = cpmp.Lattice(cpmp.make_params(*dims, **kw))
+        lat.set_wall_mu(m, hi)
+        lat.init_from_rho(rho0)
+        lat.step(nsteps)
+        rho = lat.get("rho")
+        fl = lat.get("type") == 1
+        lat.close()
+        return 0.5 * np.sum((rho[fl] - target[fl]) ** 2)
+
+    lat = cpmp.Lattice(cpmp.make_params(*dims, **kw))
+    lat.set_wall_mu(mu, hi)
+    lat.init_from_rho(rho0)
+    t0 = time.perf_counter()
+    lat.step(nsteps)
+    t_fwd = time.perf_counter() - t0
+    lat.init_from_rho(rho0)
+    t0 = time.perf_counter()
+    J, glo, ghi, gka = lat.adjoint(nsteps, target, checkpoint_every=100)
+    t_adj = time.perf_counter() - t0
+    print(f"[c4] J={J:.6e} |g|={np.linalg.norm(glo):.4e} adjoint/forward time = {t_adj / t_fwd:.2f}", flush=True)
+    assert np.isfinite(J) and J > 0
+    rng = np.random.default_rng(6)
+    h = 1e-6
+    worst = 0.0
+    for i in range(16):
+        d = rng.standard_normal(mu.size)
+        fd = (run(mu + h * d) - run(mu - h * d)) / (2 * h)
+        ad = glo @ d
+        worst = max(worst, abs(ad - fd) / abs(fd))
+        print(f"[c4-fd] dir {i}: adjoint {ad:.8e} fd {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+    assert worst < 1e-4

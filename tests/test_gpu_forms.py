@@ -83,7 +83,7 @@ def test_contact_angle_matches_oracle(cpmp):
     b = orc.contact_angle(o.p, o.get(orc.RHO), dg, dl, 0)
     assert a["valid"] == b[7] == 1
     got = np.array([a[k] for k in ("CaLeft", "CaRight", "CaCap", "XLeft", "XRight", "H", "L")])
-    assert np.allclose(got, b[:7], rtol=1e-8, atol=1e-8)
+    assert np.allclose(got, b[:7], rtol=1e-8, atol=1e-8, equal_nan=True)
 
 
 def test_write_plt_layout(cpmp, tmp_path):

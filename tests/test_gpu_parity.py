@@ -17,7 +17,7 @@ def test_fp64_rho_parity_after_n_steps(cpmp, name):
     print(f"[parity] {name} dims={case['dims']} steps={case['steps']}", flush=True)
     o, g = gc.build(case, cpmp)
     fluid = o.get(orc.TYPE) == 1
-    assert gc.rel_err(g.get("rho"), o.get(orc.RHO), fluid) < 1e-15
+    assert gc.rel_err(g.get("rho"), o.get(orc.RHO), fluid) < 1e-13   # device tanh vs glibc
     n = case["steps"]
     o.step(n)
     g.step(n)

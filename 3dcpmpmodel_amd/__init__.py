@@ -185,16 +185,16 @@ def eos_mu(eos, A, B, T, rho, mu_form=0):
     return lib.cpmp_eos_mu(EOS[eos], mu_form, A, B, T, rho)
 
 
-def selftest_collide(f, rho, u, F, ts, qp=None):
+def selftest_collide(f, rho, F, ts, qp=None):
+    """The device collision of one node evaluated on the CPU: (out, u)."""
     f = np.ascontiguousarray(f, np.float64)
-    u = np.ascontiguousarray(u, np.float64)
     F = np.ascontiguousarray(F, np.float64)
     ts = np.ascontiguousarray(ts, np.float64)
-    out = np.zeros(19)
+    out, u = np.zeros(19), np.zeros(3)
     qpp = None if qp is None else np.ascontiguousarray(qp, np.float64)
-    check(lib.cpmp_selftest_collide(_ptr(f), rho, _ptr(u), _ptr(F), _ptr(ts),
-                                    None if qpp is None else _ptr(qpp), _ptr(out)))
-    return out
+    check(lib.cpmp_selftest_collide(_ptr(f), rho, _ptr(F), _ptr(ts),
+                                    None if qpp is None else _ptr(qpp), _ptr(out), _ptr(u)))
+    return out, u
 
 
 class Lattice:

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const double *
     const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
     double v = 0;
     if (I < P.N) {
-        if (is_solid(P, (int)(I % P.nz))) {
+        if (is_solid(P, (int)((unsigned long long)I % (unsigned)P.nz))) {
             rb[I] = 0;
         } else {
             const double d = rho[I] - target[I];
@@ -67,10 +67,11 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
                                                      const double *__restrict__ mu0, double *__restrict__ fb_out,
                                                      double *__restrict__ vb_out, double *__restrict__ rb_loc,
                                                      double *__restrict__ Pf, double *__restrict__ Wf) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
     const long long N = P.N;
-    const Coord c = decompose(P, I);
+    const Coord c{nd.x, nd.y, nd.z};
     if (is_solid(P, c.z)) {
         rb_loc[I] = 0;
         for (int a = 0; a < 3; ++a) { vb_out[a * N + I] = 0; Pf[a * N + I] = 0; Wf[a * N + I] = 0; }
@@ -142,10 +143,11 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
 __global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const double *__restrict__ Pf,
                                                      const double *__restrict__ Wf, const double *__restrict__ rb_loc,
                                                      double *__restrict__ mub, double *__restrict__ rb_new) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
     const long long N = P.N;
-    const Coord c = decompose(P, I);
+    const Coord c{nd.x, nd.y, nd.z};
     int xs[7], ys[7], zs[7];
     wrap7(P, c, xs, ys, zs);
     const Stencil<double, double> st{P, xs, ys, zs};
@@ -158,10 +160,11 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const do
 __global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const double *__restrict__ rho0,
                                                 const double *__restrict__ mub, double *__restrict__ rb_new,
                                                 double *__restrict__ kapart) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
+    const NodeIdx nd = node_from_grid(P);
+    const long long I = nd.I;
     double kap = 0;
-    if (I < P.N) {
-        const Coord c = decompose(P, I);
+    if (nd.ok) {
+        const Coord c{nd.x, nd.y, nd.z};
         int xs[7], ys[7], zs[7];
         wrap7(P, c, xs, ys, zs);
         auto idx = [&](int dx, int dy, int dz) { return ((long long)xs[dx + 3] * P.ny + ys[dy + 3]) * P.nz + zs[dz + 3]; };
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const double 
         rb_new[I] += acc;
     }
     const double s = block_sum(kap);
-    if (threadIdx.x == 0) kapart[blockIdx.x] = s;
+    if (threadIdx.x == 0) kapart[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = s;
 }
 
 // wall-field gradients: solid mu = mu_w of the column
@@ -227,7 +230,7 @@ __global__ void __launch_bounds__(TPB) k_adj_wall_pass(const DevParams P, double
 
 __global__ void __launch_bounds__(TPB) k_adj_zero_solid(const DevParams P, double *__restrict__ rb) {
     const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I < P.N && is_solid(P, (int)(I % P.nz))) rb[I] = 0;
+    if (I < P.N && is_solid(P, (int)((unsigned long long)I % (unsigned)P.nz))) rb[I] = 0;
 }
 
 // acc[0] += sum(part[0..nb)) in a fixed order
@@ -258,6 +261,8 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const int C = std::min(d->checkpoint_every, d->nsteps);
     const int nseg = (d->nsteps + C - 1) / C;
     const unsigned nb = blocks_for(N), nbc = blocks_for(NXY);
+    const dim3 ng = node_grid(c->dp);
+    const long long nbg = (long long)ng.x * ng.y * ng.z;   // blocks of the node grid
     const hipStream_t s = c->stream;
     const DevParams &P = c->dp;
     const int passes = std::max(P.lo, P.hi);
@@ -266,7 +271,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     double *fbA, *fbB, *rb, *rbn, *rbl, *vbA, *vbB, *Pf, *Wf, *mub, *part, *gl, *gh, *acc, *fin_f, *fin_r;
     const size_t per_ck = 20 * (size_t)N;          // f (19N) + rho (N)
     const size_t per_tape = 4 * (size_t)N;          // rho (N) + u (3N)
-    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 * 2 + 3 + 3 + 1) + nb + 2 * NXY + 4 + 20 * (size_t)N;
+    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 * 2 + 3 + 3 + 1) + std::max<long long>(nb, nbg) + 2 * NXY + 4 + 20 * (size_t)N;
     double acc_h[2] = {0, 0};
 
     AHIP(hipSetDevice(c->p.device));
@@ -280,7 +285,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
         vbA = p; p += 3 * N; vbB = p; p += 3 * N;
         Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
-        part = p; p += nb; gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
+        part = p; p += std::max<long long>(nb, nbg); gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
         fin_f = p; p += 19 * N; fin_r = p; p += N;
     }
     AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
@@ -326,15 +331,15 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             AHIP(hipMemcpyAsync(c->rho, t0, N * sizeof(double), hipMemcpyDeviceToDevice, s));
             AHIP(launch_wall(c));
             AHIP(launch_mu(c));
-            hipLaunchKernelGGL(k_adj_collide, dim3(nb), dim3(TPB), 0, s, P, (const double *)fbA, (const double *)rb,
+            hipLaunchKernelGGL(k_adj_collide, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA, (const double *)rb,
                                (const double *)vbA, t1, t1 + N, t0, t0 + N, (const double *)c->mu, fbB, vbB, rbl, Pf, Wf);
             AHIP(hipGetLastError());
-            hipLaunchKernelGGL(k_adj_stencil, dim3(nb), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
+            hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
                                (const double *)rbl, mub, rbn);
             AHIP(hipGetLastError());
-            hipLaunchKernelGGL(k_adj_mu, dim3(nb), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, part);
+            hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, part);
             AHIP(hipGetLastError());
-            hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nb, acc + 1);
+            hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nbg, acc + 1);
             AHIP(hipGetLastError());
             if (passes > 0) {
                 hipLaunchKernelGGL(k_adj_wall_g, dim3(nbc), dim3(TPB), 0, s, P, (const double *)mub, gl, gh);

@@ -54,13 +54,16 @@ void fill_devparams(cpmp_ctx *c) {
     d.eos = p.eos; d.mu_form = p.mu_form; d.lap_points = p.lap_points; d.wall_scheme = p.wall_scheme;
     d.qp_on = (p.K1 != 0.0 || p.K2 != 0.0) ? 1 : 0;
     d.A = p.A; d.B = p.B; d.T = p.T; d.K2s = p.K * p.K; d.Ka = p.Ka; d.K1 = p.K1; d.K2 = p.K2;
-    for (int q = 0; q < 19; ++q) d.ts[q] = p.Ts[q];
+    for (int q = 0; q < 19; ++q) { d.ts[q] = p.Ts[q]; d.tsf[q] = (float)p.Ts[q]; }
     d.rho0 = p.rho0;
 }
 
 int check_params(const cpmp_params *p) {
     if (!p) return set_error(CPMP_EINVAL, "params is NULL");
     if (p->nx < 1 || p->ny < 1 || p->nz < 1) return set_error(CPMP_EINVAL, "domain %dx%dx%d", p->nx, p->ny, p->nz);
+    if (19LL * p->nx * p->ny * p->nz >= (1LL << 31))   // kernels index populations with 32-bit offsets
+        return set_error(CPMP_EINVAL, "domain %dx%dx%d too large: 19*N must be below 2^31", p->nx, p->ny, p->nz);
+    if (p->nx > 65535) return set_error(CPMP_EINVAL, "nx=%d exceeds the launch grid limit 65535", p->nx);
     if (p->nwall_lo < 0 || p->nwall_hi < 0 || p->nwall_lo + p->nwall_hi >= p->nz)
         return set_error(CPMP_EINVAL, "wall layers %d+%d do not fit nz=%d", p->nwall_lo, p->nwall_hi, p->nz);
     if (p->nwall_lo + p->nwall_hi > 0 && p->nz - p->nwall_lo - p->nwall_hi < 4)
@@ -561,10 +564,10 @@ double cpmp_algorithmic_bytes_per_node(cpmp_ctx *c) {
 
 // ---------------------------------------------------------------------------
 // CPU evaluation of the device collision code (parity self-test without a GPU)
-int cpmp_selftest_collide(const double *f, double rho, const double *u, const double *F, const double *ts,
-                          const double *qp, double *out) {
-    if (!f || !u || !F || !ts || !out) return set_error(CPMP_EINVAL, "NULL argument");
-    host_collide_fp64(f, rho, u, F, ts, qp, out);
+int cpmp_selftest_collide(const double *f, double rho, const double *F, const double *ts, const double *qp,
+                          double *out, double *u) {
+    if (!f || !F || !ts || !out || !u) return set_error(CPMP_EINVAL, "NULL argument");
+    host_collide_fp64(f, rho, F, ts, qp, out, u);
     return CPMP_OK;
 }
 

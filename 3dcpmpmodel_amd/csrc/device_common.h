@@ -33,6 +33,20 @@ __device__ __forceinline__ Coord decompose(const DevParams &P, long long I) {
     return c;
 }
 
+// Per-node kernels launch on a 3-D grid (ceil(nz/64), ceil(ny/4), nx) of 64 x 4
+// thread blocks, so coordinates come without integer division.
+struct NodeIdx { int x, y, z; long long I; bool ok; };
+__device__ __forceinline__ NodeIdx node_from_grid(const DevParams &P) {
+    NodeIdx n;
+    n.z = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    n.y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    n.x = (int)blockIdx.z;
+    n.ok = n.z < P.nz && n.y < P.ny;
+    n.I = ((long long)n.x * P.ny + n.y) * P.nz + n.z;
+    return n;
+}
+inline dim3 node_grid(const DevParams &P) { return dim3((P.nz + 63) / 64, (P.ny + 3) / 4, P.nx); }
+
 // Isotropic 6th-order gradient of Common.h:222-241 (Gradx7/y7/z7): per component
 // two axis differences and four face-diagonal differences, each the 6-point
 // central difference (-1, 9, -45, 0, 45, -9, 1)/60, weighted 2:1:1:1:1 and /360.
@@ -99,5 +113,34 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// Neighbour index offsets of one node (periodic wrap folded in) and the solid
+// flags of its z-1 / z+1 layers; slot [1+d] is the offset of coordinate + d.
+// Indices are 32-bit: the host guarantees 19 N < 2^31.
+struct Nbr {
+    int ox[3], oy[3], oz[3];
+    bool sb[3];
+};
+__device__ __forceinline__ Nbr make_nbr(const DevParams &P, int x, int y, int z) {
+    Nbr n;
+    const int pl = P.ny * P.nz;
+    n.ox[0] = (wrap_off(x, -1, P.nx) - x) * pl; n.ox[1] = 0; n.ox[2] = (wrap_off(x, 1, P.nx) - x) * pl;
+    n.oy[0] = (wrap_off(y, -1, P.ny) - y) * P.nz; n.oy[1] = 0; n.oy[2] = (wrap_off(y, 1, P.ny) - y) * P.nz;
+    const int zn = wrap_off(z, -1, P.nz), zp = wrap_off(z, 1, P.nz);
+    n.oz[0] = zn - z; n.oz[1] = 0; n.oz[2] = zp - z;
+    n.sb[0] = is_solid(P, zn); n.sb[1] = false; n.sb[2] = is_solid(P, zp);
+    return n;
+}
+// Storage convention ("pull"): population q is stored post-collision at the node
+// that emitted it.  The value arriving at fluid node I along q comes from
+// I - c_q, or, when I - c_q is solid, is opp(q) bounced back at I (half-way
+// bounce-back).
+__device__ __forceinline__ unsigned pull_index(const Nbr &n, unsigned Nu, unsigned Iu, int q) {
+    return n.sb[1 - cz(q)] ? (unsigned)opp(q) * Nu + Iu
+                           : (unsigned)q * Nu + (unsigned)((int)Iu + n.ox[1 - cx(q)] + n.oy[1 - cy(q)] + n.oz[1 - cz(q)]);
+}
+// Over fluid nodes and all q, pull_index is one-to-one and never lands on a
+// solid node's slot, so writing a post-stream value f_q(I) to pull_index(I, q)
+// ("un-streaming") is conflict-free; solid nodes keep their own slots q*N + I.
 
 }  // namespace cpmp

@@ -21,6 +21,7 @@ struct DevParams {
     double A, B, T, K2s, Ka;    // K2s = K*K
     double K1, K2;
     double ts[19];
+    float tsf[19];              // ts in FP32 for the FP32 arithmetic forms
     double rho0;                // FP16 shift density
 };
 
@@ -79,7 +80,7 @@ hipError_t launch_type(cpmp_ctx *c, double *out);
 size_t reduce_scratch_doubles();
 
 // host-side CPU evaluation of the per-node collision (self-test hook)
-void host_collide_fp64(const double f[19], double rho, const double u[3], const double F[3],
-                       const double ts[19], const double *qp, double out[19]);
+void host_collide_fp64(const double f[19], double rho, const double F[3], const double ts[19],
+                       const double *qp, double out[19], double u[3]);
 
 }  // namespace cpmp

@@ -37,6 +37,11 @@ hipError_t dispatch(int form, Fn &&fn) {
     return hipErrorInvalidValue;
 }
 
+// relaxation rates in the arithmetic precision, read from the kernel argument
+template <class TC> __device__ __forceinline__ const TC *rates(const DevParams &P);
+template <> __device__ __forceinline__ const double *rates<double>(const DevParams &P) { return P.ts; }
+template <> __device__ __forceinline__ const float *rates<float>(const DevParams &P) { return P.tsf; }
+
 template <class TF, class TC>
 __device__ __forceinline__ TC pop_load(const TF *p, long long i, int q, double rho0) {
     if constexpr (kShifted<TF>) return (TC)p[i] + (TC)(wq(q) * rho0);
@@ -83,21 +88,34 @@ __global__ void __launch_bounds__(TPB) k_wall(const DevParams P, TS *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Chemical potential (ChemPotential, FluidCuda.cu:444-463), FP64 arithmetic in
-// every form: mu = K^2 mu_eos(clamp(rho)) - Ka * Laplacian(rho).
+// Chemical potential (ChemPotential, FluidCuda.cu:444-463):
+//   mu = K^2 mu_eos(clamp(rho)) - Ka * Laplacian(rho),
+// the Laplacian in FP64.  mu_eos is FP64 in the FP64 form and, in the FP32 forms,
+// wherever the stencil sees an interface (density contrast above 1% of the
+// node's density -- seeded bulk noise stays far below that); in bulk phases of
+// the FP32 forms it is evaluated in FP32.
 template <class TS>
 __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    const Coord c = decompose(P, I);
-    if (is_solid(P, c.z)) return;
-    int xs[7], ys[7], zs[7];
-    wrap7(P, c, xs, ys, zs);
-    auto at = [&](int dx, int dy, int dz) {
-        return (double)rho[((long long)xs[dx + 3] * P.ny + ys[dy + 3]) * P.nz + zs[dz + 3]];
-    };
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    if (is_solid(P, nd.z)) return;
+    const int I = (int)nd.I, pl = P.ny * P.nz;
+    // 32-bit offsets of coordinate + s, s = -3..3 (periodic wrap folded in)
+    int ox[7], oy[7], oz[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+        ox[s] = (wrap_off(nd.x, s - 3, P.nx) - nd.x) * pl;
+        oy[s] = (wrap_off(nd.y, s - 3, P.ny) - nd.y) * P.nz;
+        oz[s] = wrap_off(nd.z, s - 3, P.nz) - nd.z;
+    }
     const double d = (double)rho[I];
-    const double m = mu_eos(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho(d));
+    double lo = d, hi = d;
+    auto at = [&](int dx, int dy, int dz) {
+        const double v = (double)rho[I + ox[dx + 3] + oy[dy + 3] + oz[dz + 3]];
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+        return v;
+    };
     double kal;
     if (P.lap_points == 7) {   // GradD7, Common.h:243-246
         auto ax = [&](int ux, int uy, int uz) {
@@ -111,6 +129,12 @@ __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restr
         };
         kal = P.Ka * ((-d * 90 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 12);
     }
+    const double dc = clamp_rho(d);
+    double m;
+    if (sizeof(TS) == 8 || hi - lo > 1e-2 * d)
+        m = mu_eos_t<double>(P.eos, P.mu_form, P.A, P.B, P.T, dc);
+    else
+        m = (double)mu_eos_t<float>(P.eos, P.mu_form, (float)P.A, (float)P.B, (float)P.T, (float)dc);
     mu[I] = (TS)(P.K2s * m - kal);
 }
 
@@ -123,9 +147,10 @@ template <class TF, class TS, class TC, bool MV>
 __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__restrict__ fin, TF *__restrict__ fout,
                                                  const TS *__restrict__ rho, const TS *__restrict__ mu,
                                                  double *__restrict__ mv, double *__restrict__ uout) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    const Coord c = decompose(P, I);
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
+    const Coord c{nd.x, nd.y, nd.z};
     if (is_solid(P, c.z)) return;
     const long long N = P.N;
     int xs[7], ys[7], zs[7];
@@ -136,55 +161,43 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
         zs[s] = wrapc(c.z + s - 3, P.nz);
     }
     const Stencil<TS, TC> st{P, xs, ys, zs};
+    const unsigned Nu = (unsigned)N, Iu = (unsigned)I;
 
     TC f[Q];
+    {
+        const Nbr nb = make_nbr(P, c.x, c.y, c.z);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, (long long)q * N + I, q, P.rho0);
-
-    const TC r = (TC)rho[I];
-    const TC inv = TC(1) / r;
-    TC jx, jy, jz;
-    momentum(f, jx, jy, jz);
-    const TC ux = jx * inv, uy = jy * inv, uz = jz * inv;
-    if (uout) {   // velocity tape for the adjoint's recomputed forward
-        uout[I] = (double)ux;
-        uout[N + I] = (double)uy;
-        uout[2 * N + I] = (double)uz;
+        for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, pull_index(nb, Nu, Iu, q), q, P.rho0);
     }
 
+    const TC r = (TC)rho[I];
     TC gmx, gmy, gmz, grx, gry, grz;
     st.grad(mu, gmx, gmy, gmz);
     st.grad(rho, grx, gry, grz);
     const TC third = TC(1.0 / 3.0);
     const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
-    if (MV) {
-        mv[I] = (double)(ux + Fx * inv * TC(0.5));
-        mv[N + I] = (double)(uy + Fy * inv * TC(0.5));
-        mv[2 * N + I] = (double)(uz + Fz * inv * TC(0.5));
-    }
-
-    TC ts[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) ts[q] = (TC)P.ts[q];
-    TC out[Q];
+    TC out[Q], u[3];
     if (P.qp_on) {
         TC qp[Q];
         qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
-        collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, qp, out);
+        collide(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
     } else {
-        collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, (const TC *)nullptr, out);
+        collide(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
     }
-
+    if (uout) {   // velocity tape for the adjoint's recomputed forward
+        uout[I] = (double)u[0];
+        uout[N + I] = (double)u[1];
+        uout[2 * N + I] = (double)u[2];
+    }
+    if (MV) {
+        const TC h = TC(0.5) / r;
+        mv[I] = (double)(u[0] + Fx * h);
+        mv[N + I] = (double)(u[1] + Fy * h);
+        mv[2 * N + I] = (double)(u[2] + Fz * h);
+    }
+    // post-collision values stay at this node; the next step pulls them
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int tz = zs[3 + cz(q)];
-        if (is_solid(P, tz)) {
-            fout[(long long)opp(q) * N + I] = pop_val<TF, TC>(out[q], opp(q), P.rho0);
-        } else {
-            const long long T = ((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz;
-            fout[(long long)q * N + T] = pop_val<TF, TC>(out[q], q, P.rho0);
-        }
-    }
+    for (int q = 0; q < Q; ++q) fout[(unsigned)q * Nu + Iu] = pop_val<TF, TC>(out[q], q, P.rho0);
 }
 
 // ---------------------------------------------------------------------------
@@ -237,80 +250,89 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
     const int y = y0 + ty, z = z0 + tz;
     const bool active = (y < P.ny) && (z < P.nz) && !is_solid(P, z);
 
-    auto load_plane = [&](int xg, int slot) {
-        const int xw = wrapc(xg, P.nx);
-        for (int e = threadIdx.x; e < MLP; e += TPB) {
-            const int ly = e / MLZ, lz = e - ly * MLZ;
-            const long long gi = ((long long)xw * P.ny + wrapc(y0 - MH + ly, P.ny)) * P.nz + wrapc(z0 - MH + lz, P.nz);
-            s_rho[slot][e] = rho[gi];
-            s_mu[slot][e] = mu[gi];
-        }
+    // Halo entries of one plane owned by this thread (e = tid + h*TPB) and their
+    // in-plane offsets; the plane index is added per x.
+    constexpr int HE = (MLP + TPB - 1) / TPB;
+    int hoff[HE];
+#pragma unroll
+    for (int h = 0; h < HE; ++h) {
+        const int e = threadIdx.x + h * TPB;
+        const int ly = e / MLZ, lz = e - ly * MLZ;
+        hoff[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) * P.nz + wrapc(z0 - MH + lz, P.nz) : 0;
+    }
+    TS hr[HE], hm[HE];
+    auto fetch_plane = [&](int xg) {        // global -> registers
+        const long long pb = (long long)wrapc(xg, P.nx) * P.ny * P.nz;
+#pragma unroll
+        for (int h = 0; h < HE; ++h)
+            if (threadIdx.x + h * TPB < MLP) { hr[h] = rho[pb + hoff[h]]; hm[h] = mu[pb + hoff[h]]; }
     };
-    for (int k = -MH; k < MH; ++k) load_plane(xa + k, k + MH);
+    auto store_plane = [&](int slot) {      // registers -> LDS ring slot
+#pragma unroll
+        for (int h = 0; h < HE; ++h)
+            if (threadIdx.x + h * TPB < MLP) { s_rho[slot][threadIdx.x + h * TPB] = hr[h]; s_mu[slot][threadIdx.x + h * TPB] = hm[h]; }
+    };
+    for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k); store_plane(k + MH); }
 
     const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
-    const int yp = wrap_off(ym, 1, P.ny), yn = wrap_off(ym, -1, P.ny);
-    const int zp = wrap_off(zm, 1, P.nz), zn = wrap_off(zm, -1, P.nz);
     const int base = (ty + MH) * MLZ + tz + MH;
-    TC ts[Q];
+    // populations arriving at (xg, ym, zm), gathered from their emitting nodes
+    auto gather = [&](int xg, TC *dst) {
+        const Nbr nb = make_nbr(P, xg, ym, zm);
+        const unsigned Iu = (unsigned)(((long long)xg * P.ny + ym) * P.nz + zm);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) ts[q] = (TC)P.ts[q];
-
+        for (int q = 0; q < Q; ++q) dst[q] = pop_load<TF, TC>(fin, pull_index(nb, (unsigned)N, Iu, q), q, P.rho0);
+    };
+    // software pipeline: the populations of plane x+1 and the halo of plane x+4
+    // are in flight while plane x computes
+    TC fnext[Q];
+    fetch_plane(xa + MH);
+    if (active) gather(xa, fnext);
     for (int x = xa; x < xb; ++x) {
         const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
-        TC f[Q];
-        if (active) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, (long long)q * N + I, q, P.rho0);
-        }
         __syncthreads();                       // every thread is done with plane x-4
-        load_plane(x + MH, (x - xa + 2 * MH) % 7);
+        store_plane((x - xa + 2 * MH) % 7);    // plane x+3
         __syncthreads();
+        TC f[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = fnext[q];
+        if (x + 1 < xb) {
+            fetch_plane(x + 1 + MH);
+            if (active) gather(x + 1, fnext);
+        }
         if (active) {
             int slot[7];
 #pragma unroll
             for (int d = 0; d < 7; ++d) slot[d] = (x - xa + d) % 7;
             const LdsStencil<TS, TC> sm{s_mu, slot, base}, sr{s_rho, slot, base};
             const TC r = sr.at(0, 0, 0);
-            const TC inv = TC(1) / r;
-            TC jx, jy, jz;
-            momentum(f, jx, jy, jz);
-            const TC ux = jx * inv, uy = jy * inv, uz = jz * inv;
-            if (uout) {
-                uout[I] = (double)ux;
-                uout[N + I] = (double)uy;
-                uout[2 * N + I] = (double)uz;
-            }
             TC gmx, gmy, gmz, grx, gry, grz;
             sm.grad(gmx, gmy, gmz);
             sr.grad(grx, gry, grz);
             const TC third = TC(1.0 / 3.0);
             const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
-            if (MV) {
-                mv[I] = (double)(ux + Fx * inv * TC(0.5));
-                mv[N + I] = (double)(uy + Fy * inv * TC(0.5));
-                mv[2 * N + I] = (double)(uz + Fz * inv * TC(0.5));
-            }
-            TC out[Q];
+            TC out[Q], u[3];
             if (P.qp_on) {
                 TC qp[Q];
                 qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
-                collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, qp, out);
+                collide(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
             } else {
-                collide(f, r, ux, uy, uz, Fx, Fy, Fz, ts, (const TC *)nullptr, out);
+                collide(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
             }
-            const int xp = wrap_off(x, 1, P.nx), xn = wrap_off(x, -1, P.nx);
+            if (uout) {
+                uout[I] = (double)u[0];
+                uout[N + I] = (double)u[1];
+                uout[2 * N + I] = (double)u[2];
+            }
+            if (MV) {
+                const TC h = TC(0.5) / r;
+                mv[I] = (double)(u[0] + Fx * h);
+                mv[N + I] = (double)(u[1] + Fy * h);
+                mv[2 * N + I] = (double)(u[2] + Fz * h);
+            }
+            // aligned stores: the next step pulls these post-collision values
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const int tzq = cz(q) > 0 ? zp : (cz(q) < 0 ? zn : zm);
-                if (is_solid(P, tzq)) {
-                    fout[(long long)opp(q) * N + I] = pop_val<TF, TC>(out[q], opp(q), P.rho0);
-                } else {
-                    const int txq = cx(q) > 0 ? xp : (cx(q) < 0 ? xn : x);
-                    const int tyq = cy(q) > 0 ? yp : (cy(q) < 0 ? yn : ym);
-                    fout[(long long)q * N + ((long long)txq * P.ny + tyq) * P.nz + tzq] = pop_val<TF, TC>(out[q], q, P.rho0);
-                }
-            }
+            for (int q = 0; q < Q; ++q) fout[(unsigned)q * (unsigned)N + (unsigned)I] = pop_val<TF, TC>(out[q], q, P.rho0);
         }
     }
 }
@@ -319,24 +341,31 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
 // Density (MacroCalculate, FluidCuda.cu:351-372): rho = sum_q f_q on fluid nodes.
 template <class TF, class TS, class TC>
 __global__ void __launch_bounds__(TPB) k_macro(const DevParams P, const TF *__restrict__ f, TS *__restrict__ rho) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    if (is_solid(P, (int)(I % P.nz))) return;
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
+    if (is_solid(P, nd.z)) return;
+    const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
     TC s = TC(0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s += pop_load<TF, TC>(f, (long long)q * P.N + I, q, P.rho0);
+    for (int q = 0; q < Q; ++q) s += pop_load<TF, TC>(f, pull_index(nb, (unsigned)P.N, (unsigned)I, q), q, P.rho0);
     rho[I] = (TS)s;
 }
 
 // velocity u = j/rho, rho = sum f (fluid nodes; 0 on solid)
 template <class TF, class TC>
 __global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *__restrict__ fp, double *__restrict__ out) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    if (is_solid(P, (int)(I % P.nz))) { out[I] = out[P.N + I] = out[2 * P.N + I] = 0; return; }
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
+    if (is_solid(P, nd.z)) { out[I] = out[P.N + I] = out[2 * P.N + I] = 0; return; }
+    const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
     TC f[Q], s = TC(0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) { f[q] = pop_load<TF, TC>(fp, (long long)q * P.N + I, q, P.rho0); s += f[q]; }
+    for (int q = 0; q < Q; ++q) {
+        f[q] = pop_load<TF, TC>(fp, pull_index(nb, (unsigned)P.N, (unsigned)I, q), q, P.rho0);
+        s += f[q];
+    }
     TC jx, jy, jz;
     momentum(f, jx, jy, jz);
     out[I] = (double)(jx / s);
@@ -348,9 +377,10 @@ __global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *_
 template <class TS, class TC>
 __global__ void __launch_bounds__(TPB) k_force(const DevParams P, const TS *__restrict__ rho, const TS *__restrict__ mu,
                                                double *__restrict__ out) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    const Coord c = decompose(P, I);
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
+    const Coord c{nd.x, nd.y, nd.z};
     if (is_solid(P, c.z)) { out[I] = out[P.N + I] = out[2 * P.N + I] = 0; return; }
     int xs[7], ys[7], zs[7];
 #pragma unroll
@@ -369,13 +399,30 @@ __global__ void __launch_bounds__(TPB) k_force(const DevParams P, const TS *__re
 
 // ---------------------------------------------------------------------------
 // Initial conditions (SetFlowField, FluidCuda.cu:199-241)
+
+// populations of a node at rest, Feq(d, 0) = w_q d (FluidCuda.cu:238), written in
+// the pull storage convention
+template <class TF>
+__device__ __forceinline__ void store_feq(const DevParams &P, TF *f, const NodeIdx &nd, double d) {
+    const unsigned Nu = (unsigned)P.N, Iu = (unsigned)nd.I;
+    if (is_solid(P, nd.z)) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[(unsigned)q * Nu + Iu] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+        return;
+    }
+    const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[pull_index(nb, Nu, Iu, q)] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+}
+
 template <class TF, class TS>
 __global__ void __launch_bounds__(TPB) k_init_reference(const DevParams P, TF *__restrict__ f, TS *__restrict__ rho,
                                                         TS *__restrict__ mu, double radius, double width, double rg,
                                                         double rl, double solid) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    const Coord c = decompose(P, I);
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
+    const Coord c{nd.x, nd.y, nd.z};
     double d;
     if (is_solid(P, c.z)) {
         d = solid;
@@ -387,16 +434,16 @@ __global__ void __launch_bounds__(TPB) k_init_reference(const DevParams P, TF *_
         d = (rl + rg) / 2 - (rl - rg) / 2 * tanh(double(r - radius) * 2 / width);
     }
     rho[I] = (TS)d;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) f[(long long)q * P.N + I] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+    store_feq(P, f, nd, d);
 }
 
 template <class TF, class TS>
 __global__ void __launch_bounds__(TPB) k_init_spheres(const DevParams P, TF *__restrict__ f, TS *__restrict__ rho,
                                                       TS *__restrict__ mu, const cpmp_init_desc D) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    const Coord c = decompose(P, I);
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
+    const Coord c{nd.x, nd.y, nd.z};
     double d;
     if (is_solid(P, c.z)) {
         d = D.solid_rho;
@@ -414,34 +461,38 @@ __global__ void __launch_bounds__(TPB) k_init_spheres(const DevParams P, TF *__r
         mu[I] = (TS)0;
     }
     rho[I] = (TS)d;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) f[(long long)q * P.N + I] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+    store_feq(P, f, nd, d);
 }
 
 template <class TF, class TS>
 __global__ void __launch_bounds__(TPB) k_feq_from_rho(const DevParams P, TF *__restrict__ f, const TS *__restrict__ rho) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    const double d = (double)rho[I];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) f[(long long)q * P.N + I] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    store_feq(P, f, nd, (double)rho[nd.I]);
 }
 
 // ---------------------------------------------------------------------------
-// conversions between the storage forms and FP64 host values
+// conversions between the storage forms and FP64 host values.  Host values are
+// post-stream populations (the reference's Dist after the swap, FluidCuda.cu:1290);
+// storage is the pull convention, so fluid nodes un-stream / gather through
+// pull_index and solid nodes keep their own slots.
 template <class TF>
 __global__ void __launch_bounds__(TPB) k_pop_from_double(const DevParams P, int q, const double *__restrict__ src,
                                                          TF *__restrict__ dst) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    dst[(long long)q * P.N + I] = pop_val<TF, double>(src[I], q, P.rho0);
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const unsigned Nu = (unsigned)P.N, Iu = (unsigned)nd.I;
+    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + Iu : pull_index(make_nbr(P, nd.x, nd.y, nd.z), Nu, Iu, q);
+    dst[slot] = pop_val<TF, double>(src[nd.I], q, P.rho0);
 }
 template <class TF>
 __global__ void __launch_bounds__(TPB) k_pop_to_double(const DevParams P, int q, const TF *__restrict__ src,
                                                        double *__restrict__ dst) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I >= P.N) return;
-    dst[I] = pop_load<TF, double>(src, (long long)q * P.N + I, q, P.rho0);
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const unsigned Nu = (unsigned)P.N, Iu = (unsigned)nd.I;
+    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + Iu : pull_index(make_nbr(P, nd.x, nd.y, nd.z), Nu, Iu, q);
+    dst[nd.I] = pop_load<TF, double>(src, slot, q, P.rho0);
 }
 template <class A, class B>
 __global__ void __launch_bounds__(TPB) k_convert(long long n, const A *__restrict__ src, B *__restrict__ dst) {
@@ -449,8 +500,8 @@ __global__ void __launch_bounds__(TPB) k_convert(long long n, const A *__restric
     if (I < n) dst[I] = (B)src[I];
 }
 __global__ void __launch_bounds__(TPB) k_type(const DevParams P, double *__restrict__ out) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I < P.N) out[I] = is_solid(P, (int)(I % P.nz)) ? 2.0 : 1.0;
+    const NodeIdx nd = node_from_grid(P);
+    if (nd.ok) out[nd.I] = is_solid(P, nd.z) ? 2.0 : 1.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -487,7 +538,7 @@ __global__ void __launch_bounds__(TPB) k_reduce(const DevParams P, const TS *__r
     double v[RED_VALS] = {0, 0, 1e300, 1e300, -1e300, 0, 0, 0, 0, 0, 0, 0};
     const long long stride = (long long)gridDim.x * TPB;
     for (long long I = (long long)blockIdx.x * TPB + threadIdx.x; I < P.N; I += stride) {
-        if (is_solid(P, (int)(I % P.nz))) continue;
+        if (is_solid(P, (int)((unsigned long long)I % (unsigned)P.nz))) continue;
         const double d = (double)rho[I];
         v[0] += d;
         if (!(d >= 0)) { v[1] += 1; v[2] = fmin(v[2], (double)I); }
@@ -538,7 +589,7 @@ hipError_t launch_wall(cpmp_ctx *c) {
 hipError_t launch_mu(cpmp_ctx *c) {
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
-        hipLaunchKernelGGL(k_mu<TS>, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu);
+        hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu);
         return hipGetLastError();
     });
 }
@@ -561,10 +612,10 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
         TF *fout = (TF *)c->f[c->cur ^ 1];
         if (collide_variant() == 1) {
             if (want_mv)
-                hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                                    fin, fout, (const TS *)c->rho, (const TS *)c->mu, c->mv, uout);
             else
-                hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+                hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                                    fin, fout, (const TS *)c->rho, (const TS *)c->mu, c->mv, uout);
         } else {
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + MXC - 1) / MXC;
@@ -586,7 +637,7 @@ hipError_t launch_macro(cpmp_ctx *c) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;
         using TC = typename decltype(fm)::TC;
-        hipLaunchKernelGGL((k_macro<TF, TS, TC>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+        hipLaunchKernelGGL((k_macro<TF, TS, TC>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                            (const TF *)c->f[c->cur], (TS *)c->rho);
         return hipGetLastError();
     });
@@ -604,7 +655,7 @@ hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, doubl
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;
-        hipLaunchKernelGGL((k_init_reference<TF, TS>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+        hipLaunchKernelGGL((k_init_reference<TF, TS>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                            (TF *)c->f[c->cur], (TS *)c->rho, (TS *)c->mu, radius, width, rg, rl, solid);
         return hipGetLastError();
     });
@@ -614,7 +665,7 @@ hipError_t launch_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;
-        hipLaunchKernelGGL((k_init_spheres<TF, TS>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+        hipLaunchKernelGGL((k_init_spheres<TF, TS>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                            (TF *)c->f[c->cur], (TS *)c->rho, (TS *)c->mu, *d);
         return hipGetLastError();
     });
@@ -624,7 +675,7 @@ hipError_t launch_feq_from_rho(cpmp_ctx *c) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;
-        hipLaunchKernelGGL((k_feq_from_rho<TF, TS>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+        hipLaunchKernelGGL((k_feq_from_rho<TF, TS>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                            (TF *)c->f[c->cur], (const TS *)c->rho);
         return hipGetLastError();
     });
@@ -633,7 +684,7 @@ hipError_t launch_feq_from_rho(cpmp_ctx *c) {
 hipError_t launch_pop_from_double(cpmp_ctx *c, int q, const double *src, void *dst) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
-        hipLaunchKernelGGL(k_pop_from_double<TF>, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, q, src, (TF *)dst);
+        hipLaunchKernelGGL(k_pop_from_double<TF>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, q, src, (TF *)dst);
         return hipGetLastError();
     });
 }
@@ -641,7 +692,7 @@ hipError_t launch_pop_from_double(cpmp_ctx *c, int q, const double *src, void *d
 hipError_t launch_pop_to_double(cpmp_ctx *c, int q, const void *src, double *dst) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
-        hipLaunchKernelGGL(k_pop_to_double<TF>, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, q,
+        hipLaunchKernelGGL(k_pop_to_double<TF>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, q,
                            (const TF *)src, dst);
         return hipGetLastError();
     });
@@ -669,7 +720,7 @@ hipError_t launch_velocity(cpmp_ctx *c, double *out3) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TC = typename decltype(fm)::TC;
-        hipLaunchKernelGGL((k_velocity<TF, TC>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp,
+        hipLaunchKernelGGL((k_velocity<TF, TC>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                            (const TF *)c->f[c->cur], out3);
         return hipGetLastError();
     });
@@ -679,14 +730,14 @@ hipError_t launch_force(cpmp_ctx *c, double *out3) {
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
         using TC = typename decltype(fm)::TC;
-        hipLaunchKernelGGL((k_force<TS, TC>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
+        hipLaunchKernelGGL((k_force<TS, TC>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
                            (const TS *)c->mu, out3);
         return hipGetLastError();
     });
 }
 
 hipError_t launch_type(cpmp_ctx *c, double *out) {
-    hipLaunchKernelGGL(k_type, dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, c->dp, out);
+    hipLaunchKernelGGL(k_type, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, out);
     return hipGetLastError();
 }
 
@@ -709,9 +760,9 @@ hipError_t launch_reduce(cpmp_ctx *c, double *host_out) {
 size_t reduce_scratch_doubles() { return (size_t)(RED_BLOCKS + 1) * RED_VALS; }
 
 // host evaluation of the collision (same code as the device path, FP64)
-void host_collide_fp64(const double f[19], double rho, const double u[3], const double F[3], const double ts[19],
-                       const double *qp, double out[19]) {
-    collide(f, rho, u[0], u[1], u[2], F[0], F[1], F[2], ts, qp, out);
+void host_collide_fp64(const double f[19], double rho, const double F[3], const double ts[19], const double *qp,
+                       double out[19], double u[3]) {
+    collide(f, rho, F[0], F[1], F[2], ts, qp, out, u);
 }
 
 }  // namespace cpmp

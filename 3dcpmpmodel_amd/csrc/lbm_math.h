@@ -157,26 +157,49 @@ CPMP_HD void momentum(const T f[Q], T &jx, T &jy, T &jz) {
 
 // Collision of GlobalCollide/LocalCollideMrt (FluidCuda.cu:245-347) in moment space:
 //   m* = m - S(m - Meq(rho,u)) + S Qp + [Meq(rho,u+F/rho) - Meq(rho,u)]
+//      = (I - S)(m - Meq(rho,u)) + Meq(rho,u+F/rho) + S Qp
 //   out = R m*
 // The bracket is M (Feq(rho,EV) - Feq(rho,u)), the exact-difference force term of
-// FluidCuda.cu:341, since M Feq(rho,v) = Meq(rho,v) for this basis.
+// FluidCuda.cu:341, since M Feq(rho,v) = Meq(rho,v) for this basis.  u = j/rho with
+// j = (m3, m5, m7) and rho the stored density; it is returned in u[3].  Each row
+// is finished as soon as its equilibrium values are formed, which keeps the
+// number of live registers low.
 template <class T>
-CPMP_HD void collide(const T f[Q], T rho, T ux, T uy, T uz, T Fx, T Fy, T Fz,
-                     const T ts[Q], const T *qp, T out[Q]) {
-    T m[Q], mu_[16], me[16];
+CPMP_HD void collide(const T f[Q], T rho, T Fx, T Fy, T Fz, const T *ts, const T *qp, T out[Q], T u[3]) {
+    T m[Q];
     moments(f, m);
-    meq(rho, ux, uy, uz, mu_);
     const T inv = T(1) / rho;
-    meq(rho, ux + Fx * inv, uy + Fy * inv, uz + Fz * inv, me);
+    const T ux = m[3] * inv, uy = m[5] * inv, uz = m[7] * inv;
+    u[0] = ux; u[1] = uy; u[2] = uz;
+    const T ex = ux + Fx * inv, ey = uy + Fy * inv, ez = uz + Fz * inv;
+    const T uu = ux * ux + uy * uy + uz * uz, ee = ex * ex + ey * ey + ez * ez;
+    const T u9 = T(2) * ux * ux - uy * uy - uz * uz, e9 = T(2) * ex * ex - ey * ey - ez * ez;
+    const T u11 = uy * uy - uz * uz, e11 = ey * ey - ez * ez;
     T a[Q];
+    auto row = [&](int r, T mequ, T meqe) {
+        T v = (T(1) - ts[r]) * (m[r] - mequ) + meqe;
+        if (qp) v += ts[r] * qp[r];
+        a[r] = v * T(1.0 / n2(r));
+    };
+    const T c23 = T(-2.0 / 3.0);
+    row(0, rho, rho);
+    row(1, rho * (T(-11) + T(19) * uu), rho * (T(-11) + T(19) * ee));
+    row(2, rho * (T(3) - T(5.5) * uu), rho * (T(3) - T(5.5) * ee));
+    row(3, rho * ux, rho * ex);
+    row(4, rho * ux * c23, rho * ex * c23);
+    row(5, rho * uy, rho * ey);
+    row(6, rho * uy * c23, rho * ey * c23);
+    row(7, rho * uz, rho * ez);
+    row(8, rho * uz * c23, rho * ez * c23);
+    row(9, rho * u9, rho * e9);
+    row(10, rho * u9 * T(-0.5), rho * e9 * T(-0.5));
+    row(11, rho * u11, rho * e11);
+    row(12, rho * u11 * T(-0.5), rho * e11 * T(-0.5));
+    row(13, rho * ux * uy, rho * ex * ey);
+    row(14, rho * uy * uz, rho * ey * ez);
+    row(15, rho * ux * uz, rho * ex * ez);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        T ms = m[r] - ts[r] * (m[r] - mu_[r]) + (me[r] - mu_[r]);
-        if (qp) ms += ts[r] * qp[r];
-        a[r] = ms * T(1.0 / n2(r));
-    }
-#pragma unroll
-    for (int r = 16; r < Q; ++r) a[r] = (m[r] - ts[r] * m[r]) * T(1.0 / n2(r));
+    for (int r = 16; r < Q; ++r) a[r] = (T(1) - ts[r]) * m[r] * T(1.0 / n2(r));
     inverse_scaled(a, out);
 }
 
@@ -209,24 +232,30 @@ CPMP_HD double clamp_drho(double d) {
 constexpr double kSqrt2 = 1.4142135623730950488016887242097;
 
 // Bulk chemical potential of each EOS (without K^2).  PR: FluidCuda.cu:453.
-// The others are the closed forms of integral p'(rho)/rho drho.
-CPMP_HD double mu_eos(int eos, int form, double A, double B, double T, double d) {
+// The others are the closed forms of integral p'(rho)/rho drho.  R is the
+// arithmetic type (double everywhere except the bulk of the FP32 forms).
+template <class R>
+CPMP_HD R mu_eos_t(int eos, int form, R A, R B, R T, R d) {
+    const R one = R(1), s2 = R(kSqrt2);
     if (form == 1 || eos == EOS_PRW || eos == EOS_PRM) {
-        const double bd = B * d;
-        return T * log(d / (1.0 - bd)) - A / (kSqrt2 * 2 * B) * log((kSqrt2 - 1 + bd) / (kSqrt2 + 1 - bd))
-             + T / (1.0 - bd) - A * d / (1.0 + bd * 2 - bd * bd);
+        const R bd = B * d;
+        return T * log(d / (one - bd)) - A / (s2 * R(2) * B) * log((s2 - one + bd) / (s2 + one - bd))
+             + T / (one - bd) - A * d / (one + bd * R(2) - bd * bd);
     }
     if (eos == EOS_VDW) {
-        const double bd = B * d;
-        return T * log(d / (1.0 - bd)) + T / (1.0 - bd) - 2.0 * A * d;
+        const R bd = B * d;
+        return T * log(d / (one - bd)) + T / (one - bd) - R(2) * A * d;
     }
     if (eos == EOS_RKE || eos == EOS_RKS) {
-        const double bd = B * d;
-        return T * log(d / (1.0 - bd)) + T / (1.0 - bd) - A / B * log(1.0 + bd) - A * d / (1.0 + bd);
+        const R bd = B * d;
+        return T * log(d / (one - bd)) + T / (one - bd) - A / B * log(one + bd) - A * d / (one + bd);
     }
     // CSE: Carnahan-Starling hard spheres with B = b/4
-    const double e = B * d, om = 1.0 - e;
-    return T * (log(d) + (8.0 * e - 9.0 * e * e + 3.0 * e * e * e) / (om * om * om)) - 2.0 * A * d;
+    const R e = B * d, om = one - e;
+    return T * (log(d) + (R(8) * e - R(9) * e * e + R(3) * e * e * e) / (om * om * om)) - R(2) * A * d;
+}
+CPMP_HD double mu_eos(int eos, int form, double A, double B, double T, double d) {
+    return mu_eos_t<double>(eos, form, A, B, T, d);
 }
 
 // d mu_eos / d rho

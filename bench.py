@@ -49,6 +49,99 @@ def time_form(cpmp, n, form, steps, warmup):
                 bytes_per_node=bpn, mass_drift=abs(s.mass - m0) / m0, max_speed=s.max_speed, bad=s.bad_count)
 
 
+G_PRW70, L_PRW70 = 0.0556211, 8.08045    # CoCurve_PRW_2.txt Tr=0.70 row times Rc
+G_PRW75, L_PRW75 = 0.108623, 7.67892     # Tr=0.75
+G_CSE85, L_CSE85 = 0.0315818, 0.279262   # CoCurve_CSE_2.txt Tr=0.85
+# wall chemical potentials of the reference's BasePt -> contact-angle table (FluidCuda.cu:128)
+WETTING = [(-0.06, 30.572), (-0.04, 49.094), (0.02, 91.975), (0.072, 120.107), (0.15, 149.187)]
+
+
+def sessile(cpmp, form, mu_w, steps):
+    """128x128x96 sessile PR droplet (Tr = 0.7): hemisphere of radius 30 on the
+    bottom wall, 3+3 wall layers; returns the contact angles and phase densities."""
+    nx, ny, nz = 128, 128, 96
+    g = cpmp.Lattice(cpmp.make_params(nx, ny, nz, eos="PRW", tr=0.7, form=form, rho0=G_PRW70))
+    g.set_wall_mu(mu_w)
+    g.init_spheres([(nx / 2, ny / 2, 2.5, 30.0)], 5.0, G_PRW70, L_PRW70, solid_rho=0.0)
+    g.step(steps)
+    ca = g.contact_angle(ny // 2, G_PRW70, L_PRW70)
+    cap = g.cap_fit(G_PRW70, L_PRW70)
+    rho = g.get("rho").reshape(nx, ny, nz)
+    out = dict(ca_left=ca["CaLeft"], ca_right=ca["CaRight"], ca_cap=ca["CaCap"], cap3d=cap["angle"],
+               rho_gas=float(rho[2, 2, nz // 2]), rho_liq=float(rho[nx // 2, ny // 2, 6]))
+    g.close()
+    return out
+
+
+def accuracy(cpmp, steps):
+    """FP32 and FP16-storage forms against the FP64 form on the same inputs."""
+    res = {"sessile_steps": steps, "cases": []}
+    worst = {"fp32": 0.0, "fp16": 0.0}
+    worst_rho = {"fp32": 0.0, "fp16": 0.0}
+    for mu_w, ref_angle in WETTING:
+        r64 = sessile(cpmp, "fp64", mu_w, steps)
+        case = {"mu_w": mu_w, "reference_table_angle": ref_angle, "fp64": r64}
+        for form in ("fp32", "fp16"):
+            r = sessile(cpmp, form, mu_w, steps)
+            err = max(abs(r[k] - r64[k]) for k in ("ca_left", "ca_right", "cap3d"))
+            rerr = max(abs(r["rho_gas"] / r64["rho_gas"] - 1), abs(r["rho_liq"] / r64["rho_liq"] - 1))
+            worst[form] = max(worst[form], err)
+            worst_rho[form] = max(worst_rho[form], rerr)
+            case[form] = r
+        res["cases"].append(case)
+        log(f"[bench] wetting mu_w={mu_w}: fp64 CaLeft {r64['ca_left']:.2f} cap3d {r64['cap3d']:.2f} "
+            f"(table {ref_angle}); fp32 err {abs(case['fp32']['ca_left'] - r64['ca_left']):.2e} deg")
+    res["contact_angle_err_deg"] = worst
+    res["coexistence_rel_err"] = worst_rho
+    # Laplace jump and spurious current: 64^3 Carnahan-Starling droplet, 1000 steps
+    lap = {}
+    for form in ("fp64", "fp32", "fp16"):
+        n = 64
+        p = cpmp.make_params(n, n, n, eos="CSE", tr=0.85, nwall=0, Ka=0.01, tau=1.0, form=form, rho0=G_CSE85)
+        g = cpmp.Lattice(p)
+        g.init_spheres([(32, 32, 32, 16)], 2.5, G_CSE85, L_CSE85, noise_amp=1e-6, seed=1)
+        g.step(1000)
+        rho = g.get("rho").reshape(n, n, n)
+        s = g.reduce()
+        A, B, T = p.A, p.B, p.T
+        dp = p.K ** 2 * (cpmp.eos_pressure("CSE", A, B, T, rho[32, 32, 32]) - cpmp.eos_pressure("CSE", A, B, T, rho[0, 0, 0]))
+        lap[form] = {"laplace_dp": dp, "spurious_max_u": s.max_speed}
+        g.close()
+    res["laplace"] = lap
+    res["laplace_rel_err"] = {f: abs(lap[f]["laplace_dp"] / lap["fp64"]["laplace_dp"] - 1) for f in ("fp32", "fp16")}
+    return res
+
+
+def adjoint_cost(cpmp, nsteps=2000, every=100):
+    """adjoint (forward with checkpoints + recompute + reverse) time over forward
+    time, 96x96x64 sessile PR droplet, Tr = 0.75, FP64."""
+    import numpy as np
+    nx, ny, nz = 96, 96, 64
+    p = cpmp.make_params(nx, ny, nz, eos="PRW", tr=0.75)
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    r = np.sqrt((x - nx / 2) ** 2 + (y - ny / 2) ** 2 + (z - 2.5) ** 2)
+    rho0 = G_PRW75 + (L_PRW75 - G_PRW75) * 0.5 * (1 - np.tanh(2 * (r - 24) / 5.0))
+    rho0[:, :, :3] = 0
+    rho0[:, :, nz - 3:] = 0
+    g = cpmp.Lattice(p)
+    g.set_wall_mu(0.0)
+    g.init_from_rho(rho0.ravel())
+    g.step(nsteps)
+    target = g.get("rho")
+    g.set_wall_mu(0.01)
+    g.init_from_rho(rho0.ravel())
+    t0 = time.perf_counter()
+    g.step(nsteps)
+    t_fwd = time.perf_counter() - t0
+    g.init_from_rho(rho0.ravel())
+    t0 = time.perf_counter()
+    J, glo, ghi, gka = g.adjoint(nsteps, target, checkpoint_every=every)
+    t_adj = time.perf_counter() - t0
+    g.close()
+    return {"config": f"{nx}x{ny}x{nz} PRW Tr=0.75, {nsteps} steps, checkpoints every {every}",
+            "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd, "J": J}
+
+
 def cpu_baseline(n, steps):
     """The FP64 OpenMP oracle (CPU restatement of the reference step) on the host
     cores, on a bounded sample: the same 256^3 workload for `steps` steps."""
@@ -83,6 +176,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,fp64,fp16")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--sessile-steps", type=int, default=4000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -105,6 +200,14 @@ def main():
                     frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None,
                     kernel="k_collide (fp32)", bytes_per_node=main_form["bytes_per_node"])
     step_bw = main_form["bytes_per_node"] * nodes / (main_form["ms_per_step"] * 1e-3) / 1e9
+    acc = adj = None
+    if not args.no_accuracy:
+        log("[bench] accuracy of the FP32 / FP16 forms against FP64 ...")
+        acc = accuracy(cpmp, args.sessile_steps)
+        log(f"[bench] contact angle err {acc['contact_angle_err_deg']}, coexistence {acc['coexistence_rel_err']}")
+        log("[bench] adjoint cost ...")
+        adj = adjoint_cost(cpmp)
+        log(f"[bench] adjoint/forward = {adj['adjoint_over_forward']:.2f}")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
@@ -133,6 +236,12 @@ def main():
                       "bytes_per_node": v["bytes_per_node"], "mass_drift": v["mass_drift"]}
                   for k, v in res.items()},
         "fp32_over_fp64": round(res["fp32"]["mlups"] / res["fp64"]["mlups"], 3) if "fp64" in res else None,
+        "contact_angle_err_deg": acc["contact_angle_err_deg"] if acc else None,
+        "coexistence_rel_err": acc["coexistence_rel_err"] if acc else None,
+        "laplace_rel_err": acc["laplace_rel_err"] if acc else None,
+        "adjoint_over_forward": round(adj["adjoint_over_forward"], 3) if adj else None,
+        "accuracy": acc,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

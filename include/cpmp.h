@@ -210,9 +210,9 @@ const char *cpmp_version(void);
    Runs the device collision code of one fluid node on the CPU in FP64
    (GlobalCollide's per-node arithmetic, FluidCuda.cu:316-344, without the
    stream): out = post-collision populations incl. the exact-difference force
-   term.  qp may be NULL (K1 = K2 = 0). */
-int cpmp_selftest_collide(const double *f19, double rho, const double *u3, const double *F3,
-                          const double *ts19, const double *qp19, double *out19);
+   term, u3 = j(f)/rho used by it.  qp may be NULL (K1 = K2 = 0). */
+int cpmp_selftest_collide(const double *f19, double rho, const double *F3, const double *ts19,
+                          const double *qp19, double *out19, double *u3);
 
 #ifdef __cplusplus
 }

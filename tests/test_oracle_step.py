@@ -180,9 +180,14 @@ def test_collide_matches_dense_brute_force(orc, cpmp):
         qp = np.zeros(19)
         if trial % 2:
             qp[[1, 9, 11, 13, 14, 15]] = rng.uniform(-1e-3, 1e-3, 6)
+        rho_s = rho * (1 + 1e-3 * rng.standard_normal())   # stored density (MacroCalculate)
+        e = np.stack([ex, ey, ez], 1)
+        u = e.T @ f / rho_s                                 # V of MacroCalculate, FluidCuda.cu:364-366
         m = M @ f
-        dense = R @ (m - S @ (m - meq(rho, u)) + S @ qp) + feq(rho, u + F / rho, ex, ey, ez, w) - feq(rho, u, ex, ey, ez, w)
-        got = cpmp.selftest_collide(f, rho, u, F, ts, qp if trial % 2 else None)
+        dense = (R @ (m - S @ (m - meq(rho_s, u)) + S @ qp) + feq(rho_s, u + F / rho_s, ex, ey, ez, w)
+                 - feq(rho_s, u, ex, ey, ez, w))
+        got, gu = cpmp.selftest_collide(f, rho_s, F, ts, qp if trial % 2 else None)
+        assert np.allclose(gu, u, rtol=1e-13, atol=1e-16)
         assert np.allclose(got, dense, rtol=1e-12, atol=1e-13 * rho)
 
 

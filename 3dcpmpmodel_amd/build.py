@@ -40,6 +40,14 @@ def build(verbose=False):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed")
     os.replace(OUT + ".tmp", OUT)
+    # reference kernels of known traffic (counter calibration, attainable bandwidth)
+    probe = os.path.join(HERE, "probe_copy")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-Wno-unused-command-line-argument", os.path.join(CSRC, "probe_copy.hip"), "-o", probe]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed on probe_copy.hip")
     return OUT
 
 

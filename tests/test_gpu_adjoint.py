@@ -115,6 +115,31 @@ def test_adjoint_matches_central_differences(cpmp):
     assert gka == pytest.approx(fd, rel=1e-4)
 
 
+def test_autograd_wrapper(cpmp):
+    """the checking wrapper: backward returns the adjoint gradient; a directional
+    derivative of the objective matches central differences."""
+    import importlib
+    import torch
+    ag = importlib.import_module("3dcpmpmodel_amd.autograd")
+    dims = (16, 12, 20)
+    kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims)
+    kw = dict(kw)
+    ka0 = kw.pop("Ka")
+    setup = dict(dims=dims, params=kw, rho0=rho0, target=target, nsteps=40, checkpoint_every=10, mu_hi=mu_hi)
+    mu = torch.tensor(mu_lo, dtype=torch.float64, requires_grad=True)
+    ka = torch.tensor(ka0, dtype=torch.float64, requires_grad=True)
+    J = ag.wall_objective(mu, ka, setup)
+    J.backward()
+    d = torch.from_numpy(np.random.default_rng(12).standard_normal(mu_lo.size))
+    h = 1e-6
+    with torch.no_grad():
+        fd = (ag.wall_objective(mu + h * d, ka, setup) - ag.wall_objective(mu - h * d, ka, setup)) / (2 * h)
+    assert float(mu.grad @ d) == pytest.approx(float(fd), rel=1e-4)
+    with torch.no_grad():
+        fdk = (ag.wall_objective(mu, ka + 1e-7, setup) - ag.wall_objective(mu, ka - 1e-7, setup)) / 2e-7
+    assert float(ka.grad) == pytest.approx(float(fdk), rel=1e-4)
+
+
 def c4_setup(cpmp, nsteps_target):
     """96x96x64 sessile PR droplet at Tr = 0.75; wall field seed 4, target from a
     forward run with the wall field of seed 5."""

@@ -36,16 +36,30 @@ __device__ __forceinline__ Coord decompose(const DevParams &P, long long I) {
 // Per-node kernels launch on a 3-D grid (ceil(nz/64), ceil(ny/4), nx) of 64 x 4
 // thread blocks, so coordinates come without integer division.
 struct NodeIdx { int x, y, z; long long I; bool ok; };
+// One block = 64 z x 4 y of one x plane; blocks are numbered z-block fastest,
+// then y-block, then x.  Hardware hands consecutive block ids to the 8 XCDs in
+// turn; when the grid divides by 8 the id is remapped so that each XCD works
+// through one contiguous run of blocks, keeping z- and y-neighbour blocks (which
+// read the same partial cache lines through the shifted population reads) in
+// one L2.
 __device__ __forceinline__ NodeIdx node_from_grid(const DevParams &P) {
     NodeIdx n;
-    n.z = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
-    n.y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
-    n.x = (int)blockIdx.z;
+    unsigned t = blockIdx.x;
+    if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
+    const unsigned nzb = ((unsigned)P.nz + 63) >> 6, nyb = ((unsigned)P.ny + 3) >> 2;
+    const unsigned zb = t % nzb;
+    t /= nzb;
+    const unsigned yb = t % nyb;
+    n.z = (int)(zb * 64 + (threadIdx.x & 63));
+    n.y = (int)(yb * 4 + (threadIdx.x >> 6));
+    n.x = (int)(t / nyb);
     n.ok = n.z < P.nz && n.y < P.ny;
     n.I = ((long long)n.x * P.ny + n.y) * P.nz + n.z;
     return n;
 }
-inline dim3 node_grid(const DevParams &P) { return dim3((P.nz + 63) / 64, (P.ny + 3) / 4, P.nx); }
+inline dim3 node_grid(const DevParams &P) {
+    return dim3((unsigned)(((P.nz + 63) / 64) * ((P.ny + 3) / 4) * P.nx));
+}
 
 // Isotropic 6th-order gradient of Common.h:222-241 (Gradx7/y7/z7): per component
 // two axis differences and four face-diagonal differences, each the 6-point

@@ -89,11 +89,77 @@ __global__ void __launch_bounds__(TPB) k_wall(const DevParams P, TS *__restrict_
 
 // ---------------------------------------------------------------------------
 // Chemical potential (ChemPotential, FluidCuda.cu:444-463):
-//   mu = K^2 mu_eos(clamp(rho)) - Ka * Laplacian(rho),
-// the Laplacian in FP64.  mu_eos is FP64 in the FP64 form and, in the FP32 forms,
-// wherever the stencil sees an interface (density contrast above 1% of the
-// node's density -- seeded bulk noise stays far below that); in bulk phases of
-// the FP32 forms it is evaluated in FP32.
+//   mu = K^2 mu_eos(clamp(rho)) - Ka * Laplacian(rho).
+// FP64 form: all in FP64, the Laplacian as the reference writes it.
+// FP32 storage forms (mu is stored in FP32): the Laplacian is summed over the
+// differences rho_k - rho_0 (its coefficients sum to zero, so this is the same
+// operator, and the differences carry no cancellation), and mu_eos is evaluated
+// in FP32 with the hardware log2 and reciprocal (relative error of a few ulp,
+// below the FP32 rounding of the stored mu).
+
+// mu_eos in FP32 with v_log_f32 / v_rcp_f32; same closed forms as mu_eos_t
+__device__ __forceinline__ float ln_hw(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945f; }
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float mu_eos_fast(int eos, int form, float A, float B, float T, float d) {
+    const float s2 = 1.41421356f;
+    const float bd = B * d, rom = rcp_hw(1.0f - bd);
+    if (form == 1 || eos == EOS_PRW || eos == EOS_PRM) {
+        return T * ln_hw(d * rom) - A * rcp_hw(s2 * 2.0f * B) * ln_hw((s2 - 1.0f + bd) * rcp_hw(s2 + 1.0f - bd))
+               + T * rom - A * d * rcp_hw(1.0f + bd * 2.0f - bd * bd);
+    }
+    if (eos == EOS_VDW) return T * ln_hw(d * rom) + T * rom - 2.0f * A * d;
+    if (eos == EOS_RKE || eos == EOS_RKS) {
+        const float op = 1.0f + bd;
+        return T * ln_hw(d * rom) + T * rom - A * rcp_hw(B) * ln_hw(op) - A * d * rcp_hw(op);
+    }
+    return T * (ln_hw(d) + (8.0f * bd - 9.0f * bd * bd + 3.0f * bd * bd * bd) * rom * rom * rom) - 2.0f * A * d;
+}
+
+// mu at one fluid node; get(dx, dy, dz) returns rho at the offset node
+template <class TS, class Get>
+__device__ __forceinline__ TS mu_node(const DevParams &P, Get get) {
+    if constexpr (sizeof(TS) == 8) {
+        const double d = get(0, 0, 0);
+        double kal;
+        if (P.lap_points == 7) {   // GradD7, Common.h:243-246
+            auto ax = [&](int ux, int uy, int uz) {
+                return 2 * get(-3 * ux, -3 * uy, -3 * uz) - 27 * get(-2 * ux, -2 * uy, -2 * uz)
+                       + 270 * get(-ux, -uy, -uz) + 270 * get(ux, uy, uz) - 27 * get(2 * ux, 2 * uy, 2 * uz)
+                       + 2 * get(3 * ux, 3 * uy, 3 * uz);
+            };
+            kal = P.Ka * (-d * 490 * 3 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 180;
+        } else {                   // GradD5, Common.h:189-192
+            auto ax = [&](int ux, int uy, int uz) {
+                return -get(-2 * ux, -2 * uy, -2 * uz) + get(-ux, -uy, -uz) * 16 + get(ux, uy, uz) * 16
+                       - get(2 * ux, 2 * uy, 2 * uz);
+            };
+            kal = P.Ka * ((-d * 90 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 12);
+        }
+        return P.K2s * mu_eos_t<double>(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho(d)) - kal;
+    } else {
+        const float d = get(0, 0, 0);
+        auto df = [&](int dx, int dy, int dz) { return (float)get(dx, dy, dz) - d; };
+        float lap;
+        if (P.lap_points == 7) {
+            auto ax = [&](int ux, int uy, int uz) {
+                return 2.0f * (df(-3 * ux, -3 * uy, -3 * uz) + df(3 * ux, 3 * uy, 3 * uz))
+                       - 27.0f * (df(-2 * ux, -2 * uy, -2 * uz) + df(2 * ux, 2 * uy, 2 * uz))
+                       + 270.0f * (df(-ux, -uy, -uz) + df(ux, uy, uz));
+            };
+            lap = (ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) * (1.0f / 180.0f);
+        } else {
+            auto ax = [&](int ux, int uy, int uz) {
+                return 16.0f * (df(-ux, -uy, -uz) + df(ux, uy, uz)) - (df(-2 * ux, -2 * uy, -2 * uz) + df(2 * ux, 2 * uy, 2 * uz));
+            };
+            lap = (ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) * (1.0f / 12.0f);
+        }
+        const float dc = d > 9.7f ? 9.7f + (d - 9.7f) * rcp_hw(1.0f + (d - 9.7f) * 1.5f) : d;   // clamp_rho
+        const float m = mu_eos_fast(P.eos, P.mu_form, (float)P.A, (float)P.B, (float)P.T, dc);
+        return (float)P.K2s * m - (float)P.Ka * lap;
+    }
+}
+
+// one thread per node, stencil values from the caches
 template <class TS>
 __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu) {
     const NodeIdx nd = node_from_grid(P);
@@ -108,34 +174,7 @@ __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restr
         oy[s] = (wrap_off(nd.y, s - 3, P.ny) - nd.y) * P.nz;
         oz[s] = wrap_off(nd.z, s - 3, P.nz) - nd.z;
     }
-    const double d = (double)rho[I];
-    double lo = d, hi = d;
-    auto at = [&](int dx, int dy, int dz) {
-        const double v = (double)rho[I + ox[dx + 3] + oy[dy + 3] + oz[dz + 3]];
-        lo = fmin(lo, v);
-        hi = fmax(hi, v);
-        return v;
-    };
-    double kal;
-    if (P.lap_points == 7) {   // GradD7, Common.h:243-246
-        auto ax = [&](int ux, int uy, int uz) {
-            return 2 * at(-3 * ux, -3 * uy, -3 * uz) - 27 * at(-2 * ux, -2 * uy, -2 * uz) + 270 * at(-ux, -uy, -uz)
-                   + 270 * at(ux, uy, uz) - 27 * at(2 * ux, 2 * uy, 2 * uz) + 2 * at(3 * ux, 3 * uy, 3 * uz);
-        };
-        kal = P.Ka * (-d * 490 * 3 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 180;
-    } else {                   // GradD5, Common.h:189-192
-        auto ax = [&](int ux, int uy, int uz) {
-            return -at(-2 * ux, -2 * uy, -2 * uz) + at(-ux, -uy, -uz) * 16 + at(ux, uy, uz) * 16 - at(2 * ux, 2 * uy, 2 * uz);
-        };
-        kal = P.Ka * ((-d * 90 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 12);
-    }
-    const double dc = clamp_rho(d);
-    double m;
-    if (sizeof(TS) == 8 || hi - lo > 1e-2 * d)
-        m = mu_eos_t<double>(P.eos, P.mu_form, P.A, P.B, P.T, dc);
-    else
-        m = (double)mu_eos_t<float>(P.eos, P.mu_form, (float)P.A, (float)P.B, (float)P.T, (float)dc);
-    mu[I] = (TS)(P.K2s * m - kal);
+    mu[I] = mu_node<TS>(P, [&](int dx, int dy, int dz) { return rho[I + ox[dx + 3] + oy[dy + 3] + oz[dz + 3]]; });
 }
 
 
@@ -228,6 +267,64 @@ struct LdsStencil {
     }
 };
 
+// Chemical potential as a march over the same tiles: rho of planes x-3..x+3 with an
+// in-plane halo of 3 sits in an LDS ring, so the 13 (or 19) stencil values of every
+// node come from LDS and each rho value is fetched from memory about once per tile.
+// Arithmetic of mu_node, as k_mu.
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
+                                                  int ntz, int nty) {
+    __shared__ TS s_rho[7][MLP];
+    int t = blockIdx.x;
+    if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
+    const int tzi = t % ntz;
+    t /= ntz;
+    const int tyi = t % nty;
+    const int ch = t / nty;
+    const int y0 = tyi * MTY, z0 = tzi * MTZ;
+    const int xa = ch * MXC, xb = min(xa + MXC, P.nx);
+    const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
+    const int y = y0 + ty, z = z0 + tz;
+    const bool active = (y < P.ny) && (z < P.nz) && !is_solid(P, z);
+    constexpr int HE = (MLP + TPB - 1) / TPB;
+    int hoff[HE];
+#pragma unroll
+    for (int h = 0; h < HE; ++h) {
+        const int e = threadIdx.x + h * TPB;
+        const int ly = e / MLZ, lz = e - ly * MLZ;
+        hoff[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) * P.nz + wrapc(z0 - MH + lz, P.nz) : 0;
+    }
+    TS hr[HE];
+    auto fetch_plane = [&](int xg) {
+        const long long pb = (long long)wrapc(xg, P.nx) * P.ny * P.nz;
+#pragma unroll
+        for (int h = 0; h < HE; ++h)
+            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) hr[h] = rho[pb + hoff[h]];
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int h = 0; h < HE; ++h)
+            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) s_rho[slot][threadIdx.x + h * TPB] = hr[h];
+    };
+    for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k); store_plane(k + MH); }
+    fetch_plane(xa + MH);
+    const int base = (ty + MH) * MLZ + tz + MH;
+    const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
+    for (int x = xa; x < xb; ++x) {
+        __syncthreads();
+        store_plane((x - xa + 2 * MH) % 7);
+        __syncthreads();
+        if (x + 1 < xb) fetch_plane(x + 1 + MH);
+        if (active) {
+            int slot[7];
+#pragma unroll
+            for (int dd = 0; dd < 7; ++dd) slot[dd] = (x - xa + dd) % 7;
+            mu[((long long)x * P.ny + ym) * P.nz + zm] =
+                mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + 3]][base + dy * MLZ + dz]; });
+        }
+    }
+}
+
 template <class TF, class TS, class TC, bool MV>
 __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
@@ -265,12 +362,12 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
         const long long pb = (long long)wrapc(xg, P.nx) * P.ny * P.nz;
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if (threadIdx.x + h * TPB < MLP) { hr[h] = rho[pb + hoff[h]]; hm[h] = mu[pb + hoff[h]]; }
+            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) { hr[h] = rho[pb + hoff[h]]; hm[h] = mu[pb + hoff[h]]; }
     };
     auto store_plane = [&](int slot) {      // registers -> LDS ring slot
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if (threadIdx.x + h * TPB < MLP) { s_rho[slot][threadIdx.x + h * TPB] = hr[h]; s_mu[slot][threadIdx.x + h * TPB] = hm[h]; }
+            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) { s_rho[slot][threadIdx.x + h * TPB] = hr[h]; s_mu[slot][threadIdx.x + h * TPB] = hm[h]; }
     };
     for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k); store_plane(k + MH); }
 
@@ -586,21 +683,34 @@ hipError_t launch_wall(cpmp_ctx *c) {
     });
 }
 
+// CPMP_COLLIDE=v1 / CPMP_MU=v1 select the one-thread-per-node kernels (stencils
+// from cache) instead of the LDS marches
+static int variant(const char *var) {
+    const char *e = std::getenv(var);
+    return (e && std::strcmp(e, "v1") == 0) ? 1 : 2;
+}
+static int collide_variant() {
+    static const int v = variant("CPMP_COLLIDE");
+    return v;
+}
+static int mu_variant() {
+    static const int v = variant("CPMP_MU");
+    return v;
+}
+
 hipError_t launch_mu(cpmp_ctx *c) {
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
-        hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu);
+        if (mu_variant() == 1) {
+            hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
+                               (TS *)c->mu);
+        } else {
+            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + MXC - 1) / MXC;
+            hipLaunchKernelGGL(k_mu_march<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
+                               (const TS *)c->rho, (TS *)c->mu, ntz, nty);
+        }
         return hipGetLastError();
     });
-}
-
-// CPMP_COLLIDE=v1 selects the one-thread-per-node kernel (stencils from cache)
-static int collide_variant() {
-    static const int v = [] {
-        const char *e = std::getenv("CPMP_COLLIDE");
-        return (e && std::strcmp(e, "v1") == 0) ? 1 : 2;
-    }();
-    return v;
 }
 
 hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {

@@ -6,6 +6,10 @@
 //   pull  : out[q][I] = in[q][I - c_q (wrapped)] (shifted reads, aligned writes)
 //   push  : out[q][I + c_q (wrapped)] = in[q][I] (aligned reads, shifted writes)
 //   sum19 : out[I] = sum_q in[q][I]              (read-only, 20*4 B per node)
+//   pullx : pull on a 1-D grid whose block ids are remapped so that each XCD
+//           takes one contiguous run of z-fastest blocks (shared partial lines
+//           of the shifted reads stay in one L2)
+//   sumpx : sum over the 19 pulled values (the density pass), remapped grid
 // Prints one line per pattern: ms per launch and GB/s of the known traffic.
 //   probe_copy [reps]
 #include <hip/hip_runtime.h>
@@ -42,6 +46,29 @@ __global__ void __launch_bounds__(256) k_pattern(const float *__restrict__ in, f
     }
 }
 
+// MODE 0 pull copy, 1 pulled sum; grid of NZ/64 * NY/4 * NX blocks
+template <int MODE>
+__global__ void __launch_bounds__(256) k_pull_xcd(const float *__restrict__ in, float *__restrict__ out) {
+    unsigned t = blockIdx.x;
+    t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
+    const int z = (t % (NZ / 64)) * 64 + (threadIdx.x & 63);
+    t /= NZ / 64;
+    const int y = (t % (NY / 4)) * 4 + (threadIdx.x >> 6), x = t / (NY / 4);
+    const unsigned I = ((unsigned)x * NY + y) * NZ + z;
+    float v[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) v[q] = in[(unsigned)q * N + shifted(x, y, z, q, -1)];
+    if (MODE == 1) {
+        float s = 0;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) s += v[q];
+        out[I] = s;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) out[(unsigned)q * N + I] = v[q];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_sum19(const float *__restrict__ in, float *__restrict__ out) {
     const int z = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6), x = blockIdx.z;
     const unsigned I = ((unsigned)x * NY + y) * NZ + z;
@@ -66,8 +93,9 @@ int main(int argc, char **argv) {
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    const char *names[4] = {"copy", "push", "pull", "sum19"};
-    for (int mode = 0; mode < 4; ++mode) {
+    const char *names[6] = {"copy", "push", "pull", "sum19", "pullx", "sumpx"};
+    const dim3 g1((NZ / 64) * (NY / 4) * NX);
+    for (int mode = 0; mode < 6; ++mode) {
         float ms = 0;
         for (int pass = 0; pass < 2; ++pass) {   // first pass warms up
             hipEventRecord(e0);
@@ -76,12 +104,14 @@ int main(int argc, char **argv) {
                 if (mode == 1) hipLaunchKernelGGL(k_pattern<1>, g, t, 0, 0, a, b);
                 if (mode == 2) hipLaunchKernelGGL(k_pattern<2>, g, t, 0, 0, a, b);
                 if (mode == 3) hipLaunchKernelGGL(k_sum19, g, t, 0, 0, a, b);
+                if (mode == 4) hipLaunchKernelGGL(k_pull_xcd<0>, g1, t, 0, 0, a, b);
+                if (mode == 5) hipLaunchKernelGGL(k_pull_xcd<1>, g1, t, 0, 0, a, b);
             }
             hipEventRecord(e1);
             hipEventSynchronize(e1);
             hipEventElapsedTime(&ms, e0, e1);
         }
-        const double bytes = mode == 3 ? 20.0 * 4 * N : 2.0 * 19 * 4 * N;
+        const double bytes = (mode == 3 || mode == 5) ? 20.0 * 4 * N : 2.0 * 19 * 4 * N;
         std::printf("%-6s %9.4f ms/launch %8.1f GB/s  known bytes/launch %.0f\n", names[mode], ms / reps,
                     bytes / (ms / reps * 1e-3) / 1e9, bytes);
     }

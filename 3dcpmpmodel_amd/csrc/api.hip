@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -222,6 +223,15 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
     if (hipMalloc(&c->f[1], popset) != hipSuccess) return fail(CPMP_ENOMEM, "populations B");
     if (hipMalloc(&c->rho, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "rho");
     if (hipMalloc(&c->mu, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "mu");
+    const char *fuse = std::getenv("CPMP_FUSE");
+    size_t ring_elems = 0, chunk_elems = 0;
+    if (c->form != CPMP_FORM_FP64 && !(fuse && std::strcmp(fuse, "0") == 0) &&
+        fuse_geometry(p->nx, p->ny, p->nz, &ring_elems, &chunk_elems)) {
+        if (hipMalloc(&c->rho_next, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "next rho");
+        if (hipMalloc(&c->rbuf, ring_elems * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "tile rings");
+        if (hipMalloc(&c->cbuf, chunk_elems * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "chunk planes");
+        c->fuse_bytes = ((size_t)c->N + ring_elems + chunk_elems) * c->scal_bytes;
+    }
     if (hipMalloc((void **)&c->mv, 3 * c->N * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "MV");
     if (hipMalloc((void **)&c->mu_w, 2 * c->NXY * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "wall mu");
     if (hipMalloc((void **)&c->red, reduce_scratch_doubles() * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "reduce");
@@ -229,6 +239,7 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
     if (hipMemsetAsync(c->f[0], 0, popset, c->stream) != hipSuccess || hipMemsetAsync(c->f[1], 0, popset, c->stream) != hipSuccess ||
         hipMemsetAsync(c->rho, 0, c->N * c->scal_bytes, c->stream) != hipSuccess ||
         hipMemsetAsync(c->mu, 0, c->N * c->scal_bytes, c->stream) != hipSuccess ||
+        (c->rho_next && hipMemsetAsync(c->rho_next, 0, c->N * c->scal_bytes, c->stream) != hipSuccess) ||
         hipMemsetAsync(c->mv, 0, 3 * c->N * sizeof(double), c->stream) != hipSuccess ||
         hipMemsetAsync(c->mu_w, 0, 2 * c->NXY * sizeof(double), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
@@ -240,7 +251,7 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
 int cpmp_destroy(cpmp_ctx *c) {
     if (!c) return CPMP_OK;
     if (c->stream) { hipSetDevice(c->p.device); hipStreamSynchronize(c->stream); }
-    for (void *ptr : {c->f[0], c->f[1], c->rho, c->mu, (void *)c->mv, (void *)c->mu_w, (void *)c->red, (void *)c->staging})
+    for (void *ptr : {c->f[0], c->f[1], c->rho, c->rho_next, c->rbuf, c->cbuf, c->mu, (void *)c->mv, (void *)c->mu_w, (void *)c->red, (void *)c->staging})
         if (ptr) hipFree(ptr);
     if (c->timers) {
         for (auto &e : c->timers->ev) if (e) hipEventDestroy(e);
@@ -260,7 +271,7 @@ int cpmp_get_params(cpmp_ctx *c, cpmp_params *out) {
 int cpmp_memory(cpmp_ctx *c, int64_t *bytes) {
     if (!valid_ctx(c) || !bytes) return set_error(CPMP_EINVAL, "bad context or output");
     *bytes = (int64_t)(2 * (size_t)c->N * 19 * c->pop_bytes + 2 * c->N * c->scal_bytes + 4 * c->N * sizeof(double)
-                       + 2 * c->NXY * sizeof(double) + reduce_scratch_doubles() * sizeof(double));
+                       + 2 * c->NXY * sizeof(double) + reduce_scratch_doubles() * sizeof(double) + c->fuse_bytes);
     return CPMP_OK;
 }
 

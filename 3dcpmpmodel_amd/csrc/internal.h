@@ -47,6 +47,10 @@ struct cpmp_ctx {
     void *f[2] = {nullptr, nullptr};
     int cur = 0;
     void *rho = nullptr, *mu = nullptr;
+    // FP32 storage forms on exactly tiled lattices: density summed by the collide
+    // kernel into rho_next, tile rings and chunk partial planes for the fix-up pass
+    void *rho_next = nullptr, *rbuf = nullptr, *cbuf = nullptr;
+    size_t fuse_bytes = 0;
     double *mv = nullptr;       // 3*N, MV of the last step
     double *mu_w = nullptr;     // 2*NXY: bottom then top
     double *red = nullptr;      // reduction partials
@@ -63,6 +67,8 @@ int set_error(int code, const char *fmt, ...);
 // launchers (kernels.hip); all enqueue on ctx->stream
 hipError_t launch_wall(cpmp_ctx *c);
 hipError_t launch_mu(cpmp_ctx *c);
+bool density_fused(const cpmp_ctx *c);
+bool fuse_geometry(int nx, int ny, int nz, size_t *ring_elems, size_t *chunk_elems);
 hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout = nullptr);
 hipError_t launch_macro(cpmp_ctx *c);
 hipError_t launch_step(cpmp_ctx *c, bool want_mv);

@@ -160,12 +160,18 @@ __device__ __forceinline__ TS mu_node(const DevParams &P, Get get) {
 }
 
 // one thread per node, stencil values from the caches
+// rho_solid (when given) is the buffer the next collide sums the density into;
+// its solid nodes receive this step's wall values.
 template <class TS>
-__global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu) {
+__global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
+                                            TS *__restrict__ rho_solid) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
-    if (is_solid(P, nd.z)) return;
     const int I = (int)nd.I, pl = P.ny * P.nz;
+    if (is_solid(P, nd.z)) {
+        if (rho_solid) rho_solid[I] = rho[I];
+        return;
+    }
     // 32-bit offsets of coordinate + s, s = -3..3 (periodic wrap folded in)
     int ox[7], oy[7], oz[7];
 #pragma unroll
@@ -273,7 +279,7 @@ struct LdsStencil {
 // Arithmetic of mu_node, as k_mu.
 template <class TS>
 __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
-                                                  int ntz, int nty) {
+                                                  int ntz, int nty, TS *__restrict__ rho_solid) {
     __shared__ TS s_rho[7][MLP];
     int t = blockIdx.x;
     if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
@@ -285,7 +291,8 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
     const int xa = ch * MXC, xb = min(xa + MXC, P.nx);
     const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
     const int y = y0 + ty, z = z0 + tz;
-    const bool active = (y < P.ny) && (z < P.nz) && !is_solid(P, z);
+    const bool inr = (y < P.ny) && (z < P.nz);
+    const bool active = inr && !is_solid(P, z);
     constexpr int HE = (MLP + TPB - 1) / TPB;
     int hoff[HE];
 #pragma unroll
@@ -315,28 +322,60 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
         store_plane((x - xa + 2 * MH) % 7);
         __syncthreads();
         if (x + 1 < xb) fetch_plane(x + 1 + MH);
+        const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
         if (active) {
             int slot[7];
 #pragma unroll
             for (int dd = 0; dd < 7; ++dd) slot[dd] = (x - xa + dd) % 7;
-            mu[((long long)x * P.ny + ym) * P.nz + zm] =
-                mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + 3]][base + dy * MLZ + dz]; });
+            mu[I] = mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + 3]][base + dy * MLZ + dz]; });
+        } else if (rho_solid && inr) {
+            rho_solid[I] = s_rho[(x - xa + 3) % 7][base];
         }
     }
 }
 
-template <class TF, class TS, class TC, bool MV>
+// With RS the kernel also sums the next step's density: every post-collision
+// value (as stored) is added to the node that will pull it (MacroCalculate,
+// FluidCuda.cu:351-372, summed on the emitting side).  Populations along x stay
+// in the thread's column (registers); in-plane shifts go to an LDS accumulator
+// ring of planes x-1, x, x+1 with a one-node border.  A plane is complete once
+// the plane after it has collided: its tile nodes are stored to rho_next (the
+// sums over this tile and chunk), its border ring to rbuf, and the two partial
+// planes outside the chunk (xa-1, xb) to cbuf.  k_density_fix then adds the
+// rings of the neighbouring tiles and the partial planes of the neighbouring
+// chunks, in a fixed order, with no atomics to global memory.  Requires
+// ny % MTY == nz % MTZ == nx % MXC == 0 (fuse_geometry).
+constexpr int AY = MTY + 2, AZ = MTZ + 2, AP = AY * AZ, ARING = AP - MTY * MTZ;
+
+// accumulator position (ly, lz) of ring entry r: rows ly = 0 and ly = AY-1, then
+// the columns lz = 0 and lz = AZ-1 of rows 1..MTY
+__device__ __forceinline__ int ring_pos(int r) {
+    if (r < AZ) return r;
+    if (r < 2 * AZ) return (AY - 1) * AZ + (r - AZ);
+    return (1 + ((r - 2 * AZ) >> 1)) * AZ + (((r - 2 * AZ) & 1) ? AZ - 1 : 0);
+}
+__device__ __forceinline__ int ring_index(int ly, int lz) {
+    if (ly == 0) return lz;
+    if (ly == AY - 1) return AZ + lz;
+    return 2 * AZ + 2 * (ly - 1) + (lz == AZ - 1 ? 1 : 0);
+}
+
+template <class TF, class TS, class TC, bool MV, bool RS>
 __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
-                                                       double *__restrict__ uout, int ntz, int nty) {
+                                                       double *__restrict__ uout, int ntz, int nty,
+                                                       TS *__restrict__ rho_next, TS *__restrict__ rbuf,
+                                                       TS *__restrict__ cbuf) {
     __shared__ TS s_rho[7][MLP];
     __shared__ TS s_mu[7][MLP];
+    __shared__ TC s_mail[RS ? Q - 3 : 1][RS ? TPB : 1];   // in-plane populations 3..18 of plane x
     const long long N = P.N;
     // XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs,
     // so give each XCD a contiguous range of tiles (shared halos stay in its L2).
     int t = blockIdx.x;
     if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
+    const int tile = t;
     const int tzi = t % ntz;
     t /= ntz;
     const int tyi = t % nty;
@@ -385,6 +424,61 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
     TC fnext[Q];
     fetch_plane(xa + MH);
     if (active) gather(xa, fnext);
+
+    // density accumulation (RS).  Populations 0..2 stay in the node's column
+    // (registers acc_*, planes x-1, x, x+1); populations 3..18 leave the column:
+    // each node posts them in the mailbox s_mail (0 where bounced back or solid)
+    // and every node, and every ring entry of the tile border (threads < ARING,
+    // registers rng_*), collects what arrives from this tile in a fixed order.
+    const int own = (ty + 1) * AZ + tz + 1;
+    const int rpos = ring_pos(threadIdx.x < ARING ? threadIdx.x : 0);
+    const int rly = rpos / AZ, rlz = rpos - rly * AZ;
+    const bool bb_m = is_solid(P, wrap_off(zm, -1, P.nz)), bb_p = is_solid(P, wrap_off(zm, 1, P.nz));
+    TC acc_prev = TC(0), acc_cur = TC(0), acc_next = TC(0);
+    TC rng_prev = TC(0), rng_cur = TC(0), rng_next = TC(0);
+    if (RS && !active) {
+#pragma unroll
+        for (int q = 3; q < Q; ++q) s_mail[q - 3][threadIdx.x] = TC(0);
+    }
+    // arrivals from the mailbox of plane x at this thread's own node: sources
+    // outside the tile read a valid slot and contribute 0 (no branches)
+    const bool sy_lo = ty >= 1, sy_hi = ty <= MTY - 2, sz_lo = tz >= 1, sz_hi = tz <= MTZ - 2;
+    auto collect_own = [&](TC &pm, TC &pc, TC &pn) {
+#pragma unroll
+        for (int q = 3; q < Q; ++q) {
+            const bool ok = (cy(q) > 0 ? sy_lo : cy(q) < 0 ? sy_hi : true) && (cz(q) > 0 ? sz_lo : cz(q) < 0 ? sz_hi : true);
+            const int src = ok ? (int)threadIdx.x - cy(q) * MTZ - cz(q) : (int)threadIdx.x;
+            const TC v = ok ? s_mail[q - 3][src] : TC(0);
+            if (cx(q) == 0) pc += v;
+            else if (cx(q) > 0) pn += v;
+            else pm += v;
+        }
+    };
+    // arrivals at a ring position (ly, lz) of the tile border (accumulator frame)
+    auto collect = [&](int ly, int lz, TC &pm, TC &pc, TC &pn) {
+#pragma unroll
+        for (int q = 3; q < Q; ++q) {
+            const int sy = ly - 1 - cy(q), sz = lz - 1 - cz(q);
+            if (sy >= 0 && sy < MTY && sz >= 0 && sz < MTZ) {
+                const TC v = s_mail[q - 3][sy * MTZ + sz];
+                if (cx(q) == 0) pc += v;
+                else if (cx(q) > 0) pn += v;
+                else pm += v;
+            }
+        }
+    };
+    // plane xp (xa-1 <= xp <= xb) is complete over this tile and chunk: write it out
+    auto flush = [&](int xp, TC own_v, TC ring_v) {
+        if (xp >= xa && xp < xb) {
+            if (active) rho_next[((long long)xp * P.ny + y) * P.nz + z] = (TS)own_v;
+            if (threadIdx.x < ARING) rbuf[((long long)tile * MXC + (xp - xa)) * ARING + threadIdx.x] = (TS)ring_v;
+        } else {
+            TS *cb = cbuf + ((long long)tile * 2 + (xp < xa ? 0 : 1)) * AP;
+            cb[own] = (TS)own_v;
+            if (threadIdx.x < ARING) cb[rpos] = (TS)ring_v;
+        }
+    };
+
     for (int x = xa; x < xb; ++x) {
         const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
         __syncthreads();                       // every thread is done with plane x-4
@@ -429,9 +523,139 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
             }
             // aligned stores: the next step pulls these post-collision values
 #pragma unroll
-            for (int q = 0; q < Q; ++q) fout[(unsigned)q * (unsigned)N + (unsigned)I] = pop_val<TF, TC>(out[q], q, P.rho0);
+            for (int q = 0; q < Q; ++q) {
+                const TF sv = pop_val<TF, TC>(out[q], q, P.rho0);
+                // streaming store: the populations are read again only next step,
+                // after far more data than the caches hold has passed through
+                // (2-byte values keep ordinary stores, which merge in the cache)
+                if constexpr (sizeof(TF) >= 4) __builtin_nontemporal_store(sv, &fout[(unsigned)q * (unsigned)N + (unsigned)I]);
+                else fout[(unsigned)q * (unsigned)N + (unsigned)I] = sv;
+                if (RS) {
+                    // the value the next step pulls: as stored (and decoded)
+                    TC w;
+                    if constexpr (kShifted<TF>) w = (TC)sv + (TC)(wq(q) * P.rho0);
+                    else w = (TC)sv;
+                    const int qx = cx(q), qy = cy(q), qz = cz(q);
+                    if (qy == 0 && qz == 0) {
+                        if (qx == 0) acc_cur += w;
+                        else if (qx > 0) acc_next += w;
+                        else acc_prev += w;
+                    } else if ((qz < 0 && bb_m) || (qz > 0 && bb_p)) {
+                        acc_cur += w;                   // bounced back: pulled by this node as opp(q)
+                        s_mail[q - 3][threadIdx.x] = TC(0);
+                    } else {
+                        s_mail[q - 3][threadIdx.x] = w;
+                    }
+                }
+            }
+        }
+        if (RS) {
+            __syncthreads();                   // mailbox of plane x posted
+            collect_own(acc_prev, acc_cur, acc_next);
+            if (threadIdx.x < ARING) collect(rly, rlz, rng_prev, rng_cur, rng_next);
+            flush(x - 1, acc_prev, rng_prev);  // plane x-1 is complete
+            acc_prev = acc_cur;
+            acc_cur = acc_next;
+            acc_next = TC(0);
+            rng_prev = rng_cur;
+            rng_cur = rng_next;
+            rng_next = TC(0);
         }
     }
+    if (RS) {
+        flush(xb - 1, acc_prev, rng_prev);
+        flush(xb, acc_cur, rng_cur);
+    }
+}
+
+// Completes the density summed by k_collide_march<..., RS>: a fluid node on the
+// border of its tile receives the ring entries of the (up to three) neighbouring
+// tiles that hold it, and a node of the first / last plane of a chunk receives
+// the partial plane xb / xa-1 of the previous / next chunk (its own tile's
+// entry and the neighbouring tiles' ring entries).  Tiles and chunks wrap
+// periodically, so a lattice one tile wide reads its own ring.
+// Border items of one tile: for each of its MXC planes the ETILE border positions
+// (rows ty = 0 and MTY-1, then the z ends of the rows between), then on the first
+// and last plane the interior positions.
+constexpr int ETILE = 2 * MTZ + 2 * (MTY - 2), ITILE = (MTY - 2) * (MTZ - 2);
+constexpr int FIX_ITEMS = MXC * ETILE + 2 * ITILE;
+
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__restrict__ rho, const TS *__restrict__ rbuf,
+                                                     const TS *__restrict__ cbuf, int ntz, int nty, int nch) {
+    int t = blockIdx.x;
+    const int tzi = t % ntz;
+    t /= ntz;
+    const int tyi = t % nty;
+    const int ch = t / nty;
+    // neighbouring tiles (dy, dz in -1..1, index (dy+1)*3 + dz+1): base of their
+    // ring entries in this chunk, and of their partial planes in the previous
+    // (xb side) and next (xa-1 side) chunk
+    const int cp = (ch + nch - 1) % nch, cn = (ch + 1) % nch;
+    long long rbase[9], pbase[9], nbase[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int yy = (tyi + i / 3 - 1 + nty) % nty, zz = (tzi + i % 3 - 1 + ntz) % ntz;
+        rbase[i] = (((long long)ch * nty + yy) * ntz + zz) * MXC * ARING;
+        pbase[i] = ((((long long)cp * nty + yy) * ntz + zz) * 2 + 1) * AP;
+        nbase[i] = ((((long long)cn * nty + yy) * ntz + zz) * 2 + 0) * AP;
+    }
+    // every item of this thread at once (all loads in flight), then the updates
+    constexpr int PER = (FIX_ITEMS + TPB - 1) / TPB;
+    TS add[PER];
+    long long at[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int it = threadIdx.x + j * TPB;
+        add[j] = TS(0);
+        at[j] = -1;
+        if (it >= FIX_ITEMS) continue;
+        int k, ty, tz;
+        if (it < MXC * ETILE) {
+            k = it / ETILE;
+            const int e = it - k * ETILE;
+            if (e < MTZ) { ty = 0; tz = e; }
+            else if (e < 2 * MTZ) { ty = MTY - 1; tz = e - MTZ; }
+            else { ty = 1 + ((e - 2 * MTZ) >> 1); tz = ((e - 2 * MTZ) & 1) ? MTZ - 1 : 0; }
+        } else {
+            const int jj = it - MXC * ETILE;
+            k = jj < ITILE ? 0 : MXC - 1;
+            const int e = jj < ITILE ? jj : jj - ITILE;
+            ty = 1 + e / (MTZ - 2);
+            tz = 1 + e % (MTZ - 2);
+        }
+        const int z = tzi * MTZ + tz;
+        if (is_solid(P, z)) continue;
+        const bool ey0 = ty == 0, ey1 = ty == MTY - 1, ez0 = tz == 0, ez1 = tz == MTZ - 1;
+        const bool lo = k == 0, hi = k == MXC - 1;
+        TS a = TS(0);
+        // neighbouring tiles whose border ring holds this node (ring position ly,
+        // lz): their ring entry of this plane, and on the chunk's first / last
+        // plane their entry of the previous / next chunk's partial plane
+        auto from = [&](bool on, int dy, int dz, int ly, int lz) {
+            if (!on) return;
+            const int i = (dy + 1) * 3 + dz + 1;
+            a += rbuf[rbase[i] + k * ARING + ring_index(ly, lz)];
+            if (lo) a += cbuf[pbase[i] + ly * AZ + lz];
+            if (hi) a += cbuf[nbase[i] + ly * AZ + lz];
+        };
+        from(ey0, -1, 0, AY - 1, tz + 1);
+        from(ey1, 1, 0, 0, tz + 1);
+        from(ez0, 0, -1, ty + 1, AZ - 1);
+        from(ez1, 0, 1, ty + 1, 0);
+        from(ey0 && ez0, -1, -1, AY - 1, AZ - 1);
+        from(ey0 && ez1, -1, 1, AY - 1, 0);
+        from(ey1 && ez0, 1, -1, 0, AZ - 1);
+        from(ey1 && ez1, 1, 1, 0, 0);
+        // this tile's own entry of the neighbouring chunks' partial planes
+        if (lo) a += cbuf[pbase[4] + (ty + 1) * AZ + tz + 1];
+        if (hi) a += cbuf[nbase[4] + (ty + 1) * AZ + tz + 1];
+        add[j] = a;
+        at[j] = ((long long)(ch * MXC + k) * P.ny + tyi * MTY + ty) * P.nz + z;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+        if (at[j] >= 0) rho[at[j]] += add[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -698,51 +922,85 @@ static int mu_variant() {
     return v;
 }
 
+// FP32 storage forms with the march collide sum the density inside the collide
+// kernel when the tiles cover the lattice exactly (a context created with
+// CPMP_FUSE=0 in the environment keeps the separate density pass)
+bool fuse_geometry(int nx, int ny, int nz, size_t *ring_elems, size_t *chunk_elems) {
+    if (nx % MXC || ny % MTY || nz % MTZ) return false;
+    const size_t tiles = (size_t)(nx / MXC) * (ny / MTY) * (nz / MTZ);
+    *ring_elems = tiles * MXC * ARING;
+    *chunk_elems = tiles * 2 * AP;
+    return true;
+}
+bool density_fused(const cpmp_ctx *c) { return c->rho_next && collide_variant() == 2; }
+
 hipError_t launch_mu(cpmp_ctx *c) {
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
+        TS *solid = density_fused(c) ? (TS *)c->rho_next : nullptr;
         if (mu_variant() == 1) {
             hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
-                               (TS *)c->mu);
+                               (TS *)c->mu, solid);
         } else {
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + MXC - 1) / MXC;
             hipLaunchKernelGGL(k_mu_march<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
-                               (const TS *)c->rho, (TS *)c->mu, ntz, nty);
+                               (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid);
         }
         return hipGetLastError();
     });
 }
 
 hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
-    return dispatch(c->form, [&](auto fm) {
+    const bool rs = density_fused(c);
+    const hipError_t e = dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;
         using TC = typename decltype(fm)::TC;
         const TF *fin = (const TF *)c->f[c->cur];
         TF *fout = (TF *)c->f[c->cur ^ 1];
+        const TS *rho = (const TS *)c->rho, *mu = (const TS *)c->mu;
+        TS *rn = (TS *)c->rho_next, *rb = (TS *)c->rbuf, *cb = (TS *)c->cbuf;
         if (collide_variant() == 1) {
             if (want_mv)
                 hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, (const TS *)c->rho, (const TS *)c->mu, c->mv, uout);
+                                   fin, fout, rho, mu, c->mv, uout);
             else
                 hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, (const TS *)c->rho, (const TS *)c->mu, c->mv, uout);
+                                   fin, fout, rho, mu, c->mv, uout);
         } else {
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + MXC - 1) / MXC;
             const dim3 grid((unsigned)(ntz * nty * nch));
-            if (want_mv)
-                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true>), grid, dim3(TPB), 0, c->stream, c->dp, fin, fout,
-                                   (const TS *)c->rho, (const TS *)c->mu, c->mv, uout, ntz, nty);
+            constexpr bool can_rs = !std::is_same<TF, double>::value;   // FP64 keeps the separate pass
+            if (can_rs && rs && want_mv)
+                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
+            else if (can_rs && rs)
+                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
+            else if (want_mv)
+                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, false>), grid, dim3(TPB), 0, c->stream, c->dp,
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
             else
-                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false>), grid, dim3(TPB), 0, c->stream, c->dp, fin, fout,
-                                   (const TS *)c->rho, (const TS *)c->mu, c->mv, uout, ntz, nty);
+                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, false>), grid, dim3(TPB), 0, c->stream, c->dp,
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
         }
-        c->cur ^= 1;
         return hipGetLastError();
     });
+    c->cur ^= 1;
+    if (rs) std::swap(c->rho, c->rho_next);   // the summed density is now current
+    return e;
 }
 
 hipError_t launch_macro(cpmp_ctx *c) {
+    if (density_fused(c)) {   // summed by the collide kernel; add the tile and chunk borders
+        return dispatch(c->form, [&](auto fm) {
+            using TS = typename decltype(fm)::TS;
+            const int ntz = c->p.nz / MTZ, nty = c->p.ny / MTY, nch = c->p.nx / MXC;
+            hipLaunchKernelGGL(k_density_fix<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
+                               (TS *)c->rho, (const TS *)c->rbuf, (const TS *)c->cbuf, ntz, nty, nch);
+            return hipGetLastError();
+        });
+    }
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;

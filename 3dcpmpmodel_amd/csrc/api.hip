@@ -231,6 +231,10 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
         if (hipMalloc(&c->rbuf, ring_elems * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "tile rings");
         if (hipMalloc(&c->cbuf, chunk_elems * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "chunk planes");
         c->fuse_bytes = ((size_t)c->N + ring_elems + chunk_elems) * c->scal_bytes;
+        // the fused path stores populations tile-contiguous (pop_site)
+        c->dp.tiled = 1;
+        c->dp.nty = p->ny / kTileY;
+        c->dp.ntz = p->nz / kTileZ;
     }
     if (hipMalloc((void **)&c->mv, 3 * c->N * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "MV");
     if (hipMalloc((void **)&c->mu_w, 2 * c->NXY * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "wall mu");
@@ -450,7 +454,8 @@ int cpmp_reduce(cpmp_ctx *c, cpmp_stats *out) {
 }
 
 // ---------------------------------------------------------------------------
-// Checkpoint: magic, version, params, step, cur populations in storage form, rho.
+// Checkpoint: magic, version, params, step, cur populations in storage form (in
+// the reference node order), rho.
 static const char kMagic[8] = {'C', 'P', 'M', 'P', 'C', 'K', 'P', '1'};
 
 int cpmp_checkpoint_save(cpmp_ctx *c, const char *path) {
@@ -458,7 +463,13 @@ int cpmp_checkpoint_save(cpmp_ctx *c, const char *path) {
     HIPCHK(hipSetDevice(c->p.device));
     const size_t popset = (size_t)c->N * 19 * c->pop_bytes, sc = (size_t)c->N * c->scal_bytes;
     std::vector<char> buf(popset + sc + 2 * c->NXY * sizeof(double));
-    HIPCHK(hipMemcpyAsync(buf.data(), c->f[c->cur], popset, hipMemcpyDeviceToHost, c->stream));
+    // populations in the reference node order whatever the device layout
+    const void *pops = c->f[c->cur];
+    if (c->dp.tiled) {
+        HIPCHK(launch_pop_relayout(c, c->f[c->cur], c->f[c->cur ^ 1], false));
+        pops = c->f[c->cur ^ 1];
+    }
+    HIPCHK(hipMemcpyAsync(buf.data(), pops, popset, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(buf.data() + popset, c->rho, sc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(buf.data() + popset + sc, c->mu_w, 2 * c->NXY * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -496,7 +507,12 @@ int cpmp_checkpoint_load(cpmp_ctx *c, const char *path) {
     std::fclose(fp);
     if (!ok) return set_error(CPMP_EIO, "%s is truncated", path);
     HIPCHK(hipSetDevice(c->p.device));
-    HIPCHK(hipMemcpyAsync(c->f[c->cur], buf.data(), popset, hipMemcpyHostToDevice, c->stream));
+    if (c->dp.tiled) {
+        HIPCHK(hipMemcpyAsync(c->f[c->cur ^ 1], buf.data(), popset, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_pop_relayout(c, c->f[c->cur ^ 1], c->f[c->cur], true));
+    } else {
+        HIPCHK(hipMemcpyAsync(c->f[c->cur], buf.data(), popset, hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(hipMemcpyAsync(c->rho, buf.data() + popset, sc, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->mu_w, buf.data() + popset + sc, 2 * c->NXY * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -128,33 +128,53 @@ __device__ __forceinline__ double block_sum(double v) {
 
 inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
-// Neighbour index offsets of one node (periodic wrap folded in) and the solid
-// flags of its z-1 / z+1 layers; slot [1+d] is the offset of coordinate + d.
-// Indices are 32-bit: the host guarantees 19 N < 2^31.
+// Population storage site of node (x, y, z).  Reference order (x*ny + y)*nz + z,
+// or, on lattices the 8 x 32 (y, z) tiles cover exactly in the FP32 storage
+// forms, tile-contiguous within each x plane: the 256 nodes of a tile (y
+// slowest, then z) are one contiguous run, tiles ordered y-tile slowest.  A
+// workgroup of the march collide then streams 1 KB runs per population and
+// plane.  rho, mu and all host-facing arrays keep the reference order.
+constexpr int TILE_Y = kTileY, TILE_Z = kTileZ;
+__device__ __forceinline__ unsigned pop_site(const DevParams &P, int x, int y, int z) {
+    if (!P.tiled) return ((unsigned)x * P.ny + y) * P.nz + z;
+    return (unsigned)x * (unsigned)(P.ny * P.nz) +
+           (((unsigned)(y / TILE_Y) * P.ntz + (unsigned)(z / TILE_Z)) * (TILE_Y * TILE_Z) + (y % TILE_Y) * TILE_Z + z % TILE_Z);
+}
+
+// Storage site of one node, offsets of its neighbour sites (periodic wrap folded
+// in; [1+dx] along x, [(1+dy)*3 + 1+dz] in the plane) and the solid flags of its
+// z-1 / z+1 layers.  Indices are 32-bit: the host guarantees 19 N < 2^31.
 struct Nbr {
-    int ox[3], oy[3], oz[3];
+    unsigned site;
+    int ox[3], oyz[9];
     bool sb[3];
 };
 __device__ __forceinline__ Nbr make_nbr(const DevParams &P, int x, int y, int z) {
     Nbr n;
     const int pl = P.ny * P.nz;
+    n.site = pop_site(P, x, y, z);
     n.ox[0] = (wrap_off(x, -1, P.nx) - x) * pl; n.ox[1] = 0; n.ox[2] = (wrap_off(x, 1, P.nx) - x) * pl;
-    n.oy[0] = (wrap_off(y, -1, P.ny) - y) * P.nz; n.oy[1] = 0; n.oy[2] = (wrap_off(y, 1, P.ny) - y) * P.nz;
-    const int zn = wrap_off(z, -1, P.nz), zp = wrap_off(z, 1, P.nz);
-    n.oz[0] = zn - z; n.oz[1] = 0; n.oz[2] = zp - z;
-    n.sb[0] = is_solid(P, zn); n.sb[1] = false; n.sb[2] = is_solid(P, zp);
+    const int ys[3] = {wrap_off(y, -1, P.ny), y, wrap_off(y, 1, P.ny)};
+    const int zs[3] = {wrap_off(z, -1, P.nz), z, wrap_off(z, 1, P.nz)};
+    const int own = (int)pop_site(P, 0, y, z);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) n.oyz[a * 3 + b] = (int)pop_site(P, 0, ys[a], zs[b]) - own;
+    n.sb[0] = is_solid(P, zs[0]); n.sb[1] = false; n.sb[2] = is_solid(P, zs[2]);
     return n;
 }
 // Storage convention ("pull"): population q is stored post-collision at the node
 // that emitted it.  The value arriving at fluid node I along q comes from
 // I - c_q, or, when I - c_q is solid, is opp(q) bounced back at I (half-way
 // bounce-back).
-__device__ __forceinline__ unsigned pull_index(const Nbr &n, unsigned Nu, unsigned Iu, int q) {
-    return n.sb[1 - cz(q)] ? (unsigned)opp(q) * Nu + Iu
-                           : (unsigned)q * Nu + (unsigned)((int)Iu + n.ox[1 - cx(q)] + n.oy[1 - cy(q)] + n.oz[1 - cz(q)]);
+__device__ __forceinline__ unsigned pull_index(const Nbr &n, unsigned Nu, int q) {
+    return n.sb[1 - cz(q)] ? (unsigned)opp(q) * Nu + n.site
+                           : (unsigned)q * Nu + (unsigned)((int)n.site + n.ox[1 - cx(q)] + n.oyz[(1 - cy(q)) * 3 + 1 - cz(q)]);
 }
 // Over fluid nodes and all q, pull_index is one-to-one and never lands on a
 // solid node's slot, so writing a post-stream value f_q(I) to pull_index(I, q)
-// ("un-streaming") is conflict-free; solid nodes keep their own slots q*N + I.
+// ("un-streaming") is conflict-free; solid nodes keep their own slots
+// q*N + site.
 
 }  // namespace cpmp

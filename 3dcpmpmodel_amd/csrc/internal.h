@@ -23,6 +23,7 @@ struct DevParams {
     double ts[19];
     float tsf[19];              // ts in FP32 for the FP32 arithmetic forms
     double rho0;                // FP16 shift density
+    int tiled, nty, ntz;        // tile-contiguous population sites (pop_site), tile counts in y, z
 };
 
 enum Kern { KW = CPMP_K_WALL, KM = CPMP_K_MU, KC = CPMP_K_COLLIDE, KR = CPMP_K_MACRO };
@@ -69,6 +70,9 @@ hipError_t launch_wall(cpmp_ctx *c);
 hipError_t launch_mu(cpmp_ctx *c);
 bool density_fused(const cpmp_ctx *c);
 bool fuse_geometry(int nx, int ny, int nz, size_t *ring_elems, size_t *chunk_elems);
+constexpr int kTileY = 8, kTileZ = 32;   // in-plane tile of the march kernels and of tiled population storage
+// copy one population set between the device layout and the reference node order
+hipError_t launch_pop_relayout(cpmp_ctx *c, const void *src, void *dst, bool to_device_layout);
 hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout = nullptr);
 hipError_t launch_macro(cpmp_ctx *c);
 hipError_t launch_step(cpmp_ctx *c, bool want_mv);

@@ -206,14 +206,13 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
         zs[s] = wrapc(c.z + s - 3, P.nz);
     }
     const Stencil<TS, TC> st{P, xs, ys, zs};
-    const unsigned Nu = (unsigned)N, Iu = (unsigned)I;
+    const unsigned Nu = (unsigned)N;
+    const Nbr nb = make_nbr(P, c.x, c.y, c.z);
+    const unsigned Iu = nb.site;
 
     TC f[Q];
-    {
-        const Nbr nb = make_nbr(P, c.x, c.y, c.z);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, pull_index(nb, Nu, Iu, q), q, P.rho0);
-    }
+    for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, pull_index(nb, Nu, q), q, P.rho0);
 
     const TC r = (TC)rho[I];
     TC gmx, gmy, gmz, grx, gry, grz;
@@ -252,7 +251,7 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
 // of 3 sit in an LDS ring, so each value is fetched from L2/HBM about once per
 // tile instead of 54 times per stencil; the populations of plane x are loaded
 // (coalesced) before the ring update so their latency overlaps the barrier.
-constexpr int MTY = 8, MTZ = 32, MH = 3, MXC = 32;
+constexpr int MTY = TILE_Y, MTZ = TILE_Z, MH = 3, MXC = 32;
 constexpr int MLY = MTY + 2 * MH, MLZ = MTZ + 2 * MH, MLP = MLY * MLZ;
 
 template <class TS, class TC>
@@ -366,15 +365,15 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
                                                        double *__restrict__ uout, int ntz, int nty,
                                                        TS *__restrict__ rho_next, TS *__restrict__ rbuf,
-                                                       TS *__restrict__ cbuf) {
+                                                       TS *__restrict__ cbuf, int remap) {
     __shared__ TS s_rho[7][MLP];
     __shared__ TS s_mu[7][MLP];
     __shared__ TC s_mail[RS ? Q - 3 : 1][RS ? TPB : 1];   // in-plane populations 3..18 of plane x
     const long long N = P.N;
-    // XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs,
-    // so give each XCD a contiguous range of tiles (shared halos stay in its L2).
+    // tile order: plain, or (remap) each XCD a contiguous range of tiles --
+    // workgroups are dispatched round-robin over the 8 XCDs
     int t = blockIdx.x;
-    if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
+    if (remap && (gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
     const int tile = t;
     const int tzi = t % ntz;
     t /= ntz;
@@ -412,18 +411,27 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
 
     const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
     const int base = (ty + MH) * MLZ + tz + MH;
-    // populations arriving at (xg, ym, zm), gathered from their emitting nodes
+    const unsigned plane = (unsigned)(P.ny * P.nz), own_site = pop_site(P, 0, ym, zm);
+    // populations arriving at (xg, ym, zm), gathered from their emitting nodes;
+    // the in-plane neighbour offsets and wall flags do not change along x
+    Nbr nb = make_nbr(P, xa, ym, zm);
     auto gather = [&](int xg, TC *dst) {
-        const Nbr nb = make_nbr(P, xg, ym, zm);
-        const unsigned Iu = (unsigned)(((long long)xg * P.ny + ym) * P.nz + zm);
+        xg = wrapc(xg, P.nx);
+        nb.site = (unsigned)xg * plane + own_site;
+        nb.ox[0] = (wrap_off(xg, -1, P.nx) - xg) * (int)plane;
+        nb.ox[2] = (wrap_off(xg, 1, P.nx) - xg) * (int)plane;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) dst[q] = pop_load<TF, TC>(fin, pull_index(nb, (unsigned)N, Iu, q), q, P.rho0);
+        for (int q = 0; q < Q; ++q) dst[q] = pop_load<TF, TC>(fin, pull_index(nb, (unsigned)N, q), q, P.rho0);
     };
     // software pipeline: the populations of plane x+1 and the halo of plane x+4
     // are in flight while plane x computes
     TC fnext[Q];
+    // the loads below are issued by every thread on every path (inactive threads
+    // read valid slots of their clamped coordinates and discard them): the
+    // compiler then needs to wait only for the oldest loads at the loop head
+    // instead of draining the population stores of the previous plane
     fetch_plane(xa + MH);
-    if (active) gather(xa, fnext);
+    gather(xa, fnext);
 
     // density accumulation (RS).  Populations 0..2 stay in the node's column
     // (registers acc_*, planes x-1, x, x+1); populations 3..18 leave the column:
@@ -487,10 +495,8 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
         TC f[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = fnext[q];
-        if (x + 1 < xb) {
-            fetch_plane(x + 1 + MH);
-            if (active) gather(x + 1, fnext);
-        }
+        fetch_plane(x + 1 + MH);   // one plane past the chunk on the last pass (wrapped, unused)
+        gather(x + 1, fnext);
         if (active) {
             int slot[7];
 #pragma unroll
@@ -528,8 +534,9 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
                 // streaming store: the populations are read again only next step,
                 // after far more data than the caches hold has passed through
                 // (2-byte values keep ordinary stores, which merge in the cache)
-                if constexpr (sizeof(TF) >= 4) __builtin_nontemporal_store(sv, &fout[(unsigned)q * (unsigned)N + (unsigned)I]);
-                else fout[(unsigned)q * (unsigned)N + (unsigned)I] = sv;
+                const unsigned at = (unsigned)q * (unsigned)N + (unsigned)x * plane + own_site;
+                if constexpr (sizeof(TF) >= 4) __builtin_nontemporal_store(sv, &fout[at]);
+                else fout[at] = sv;
                 if (RS) {
                     // the value the next step pulls: as stored (and decoded)
                     TC w;
@@ -669,7 +676,7 @@ __global__ void __launch_bounds__(TPB) k_macro(const DevParams P, const TF *__re
     const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
     TC s = TC(0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s += pop_load<TF, TC>(f, pull_index(nb, (unsigned)P.N, (unsigned)I, q), q, P.rho0);
+    for (int q = 0; q < Q; ++q) s += pop_load<TF, TC>(f, pull_index(nb, (unsigned)P.N, q), q, P.rho0);
     rho[I] = (TS)s;
 }
 
@@ -684,7 +691,7 @@ __global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *_
     TC f[Q], s = TC(0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        f[q] = pop_load<TF, TC>(fp, pull_index(nb, (unsigned)P.N, (unsigned)I, q), q, P.rho0);
+        f[q] = pop_load<TF, TC>(fp, pull_index(nb, (unsigned)P.N, q), q, P.rho0);
         s += f[q];
     }
     TC jx, jy, jz;
@@ -725,15 +732,15 @@ __global__ void __launch_bounds__(TPB) k_force(const DevParams P, const TS *__re
 // the pull storage convention
 template <class TF>
 __device__ __forceinline__ void store_feq(const DevParams &P, TF *f, const NodeIdx &nd, double d) {
-    const unsigned Nu = (unsigned)P.N, Iu = (unsigned)nd.I;
+    const unsigned Nu = (unsigned)P.N;
+    const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
     if (is_solid(P, nd.z)) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[(unsigned)q * Nu + Iu] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+        for (int q = 0; q < Q; ++q) f[(unsigned)q * Nu + nb.site] = pop_val<TF, double>(d * wq(q), q, P.rho0);
         return;
     }
-    const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[pull_index(nb, Nu, Iu, q)] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+    for (int q = 0; q < Q; ++q) f[pull_index(nb, Nu, q)] = pop_val<TF, double>(d * wq(q), q, P.rho0);
 }
 
 template <class TF, class TS>
@@ -802,8 +809,9 @@ __global__ void __launch_bounds__(TPB) k_pop_from_double(const DevParams P, int 
                                                          TF *__restrict__ dst) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
-    const unsigned Nu = (unsigned)P.N, Iu = (unsigned)nd.I;
-    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + Iu : pull_index(make_nbr(P, nd.x, nd.y, nd.z), Nu, Iu, q);
+    const unsigned Nu = (unsigned)P.N;
+    const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
+    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + nb.site : pull_index(nb, Nu, q);
     dst[slot] = pop_val<TF, double>(src[nd.I], q, P.rho0);
 }
 template <class TF>
@@ -811,9 +819,24 @@ __global__ void __launch_bounds__(TPB) k_pop_to_double(const DevParams P, int q,
                                                        double *__restrict__ dst) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
-    const unsigned Nu = (unsigned)P.N, Iu = (unsigned)nd.I;
-    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + Iu : pull_index(make_nbr(P, nd.x, nd.y, nd.z), Nu, Iu, q);
+    const unsigned Nu = (unsigned)P.N;
+    const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
+    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + nb.site : pull_index(nb, Nu, q);
     dst[nd.I] = pop_load<TF, double>(src, slot, q, P.rho0);
+}
+// raw copy of every population slot between the device layout (pop_site) and the
+// reference node order
+template <class TF>
+__global__ void __launch_bounds__(TPB) k_pop_relayout(const DevParams P, const TF *__restrict__ src, TF *__restrict__ dst,
+                                                      int to_site) {
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const unsigned Nu = (unsigned)P.N, site = pop_site(P, nd.x, nd.y, nd.z), I = (unsigned)nd.I;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (to_site) dst[(unsigned)q * Nu + site] = src[(unsigned)q * Nu + I];
+        else dst[(unsigned)q * Nu + I] = src[(unsigned)q * Nu + site];
+    }
 }
 template <class A, class B>
 __global__ void __launch_bounds__(TPB) k_convert(long long n, const A *__restrict__ src, B *__restrict__ dst) {
@@ -960,6 +983,14 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
         TF *fout = (TF *)c->f[c->cur ^ 1];
         const TS *rho = (const TS *)c->rho, *mu = (const TS *)c->mu;
         TS *rn = (TS *)c->rho_next, *rb = (TS *)c->rbuf, *cb = (TS *)c->cbuf;
+        // Tiles go to the XCDs in plain round-robin order: with tile-contiguous
+        // populations the eight concurrent z tiles of a row then stream one
+        // contiguous 8 KB run per population (measured faster than giving each
+        // XCD a contiguous range of tiles); CPMP_REMAP=1 selects the ranges.
+        static const int remap = [] {
+            const char *e = std::getenv("CPMP_REMAP");
+            return (e && std::strcmp(e, "1") == 0) ? 1 : 0;
+        }();
         if (collide_variant() == 1) {
             if (want_mv)
                 hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
@@ -973,16 +1004,16 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
             constexpr bool can_rs = !std::is_same<TF, double>::value;   // FP64 keeps the separate pass
             if (can_rs && rs && want_mv)
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
             else if (can_rs && rs)
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
             else if (want_mv)
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, false>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
             else
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, false>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
         }
         return hipGetLastError();
     });
@@ -1062,6 +1093,15 @@ hipError_t launch_pop_to_double(cpmp_ctx *c, int q, const void *src, double *dst
         using TF = typename decltype(fm)::TF;
         hipLaunchKernelGGL(k_pop_to_double<TF>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, q,
                            (const TF *)src, dst);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_pop_relayout(cpmp_ctx *c, const void *src, void *dst, bool to_device_layout) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        hipLaunchKernelGGL(k_pop_relayout<TF>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TF *)src,
+                           (TF *)dst, to_device_layout ? 1 : 0);
         return hipGetLastError();
     });
 }

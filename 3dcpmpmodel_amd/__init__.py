@@ -61,7 +61,7 @@ class Stats(C.Structure):
 
 class AdjDesc(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("checkpoint_every", C.c_int),
-                ("target", C.POINTER(C.c_double))]
+                ("target", C.POINTER(C.c_double)), ("tape_budget_bytes", C.c_int64)]
 
 
 class CPMPError(RuntimeError):
@@ -293,11 +293,15 @@ class Lattice:
     def write_plt(self, path, title="Model3D", decimate=1, decimate_linear=False):
         check(lib.cpmp_write_plt(self.h, path.encode(), title.encode(), decimate, int(decimate_linear)))
 
-    def adjoint(self, nsteps, target, checkpoint_every=100, want_hi=True):
+    def adjoint(self, nsteps, target, checkpoint_every=100, want_hi=True, tape_budget_bytes=0):
+        """cpmp_adjoint.  tape_budget_bytes: trailing steps taped in the first
+        forward run instead of recomputed (0: recompute everything, -1: half of
+        the free device memory)."""
         nxy = self.p.nx * self.p.ny
         t = np.ascontiguousarray(target, np.float64).ravel()
         d = AdjDesc()
         d.nsteps, d.checkpoint_every = nsteps, checkpoint_every
+        d.tape_budget_bytes = int(tape_budget_bytes)
         d.target = t.ctypes.data_as(C.POINTER(C.c_double))
         J = np.zeros(1)
         glo, ghi, gka = np.zeros(nxy), np.zeros(nxy), np.zeros(1)

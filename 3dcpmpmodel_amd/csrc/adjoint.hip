@@ -259,7 +259,6 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     int rc = CPMP_OK;
     const long long N = c->N, NXY = c->NXY;
     const int C = std::min(d->checkpoint_every, d->nsteps);
-    const int nseg = (d->nsteps + C - 1) / C;
     const unsigned nb = blocks_for(N), nbc = blocks_for(NXY);
     const dim3 ng = node_grid(c->dp);
     const long long nbg = (long long)ng.x * ng.y * ng.z;   // blocks of the node grid
@@ -270,90 +269,112 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     double *ckpt = nullptr, *tape = nullptr, *buf = nullptr, *tgt = nullptr;
     double *fbA, *fbB, *rb, *rbn, *rbl, *vbA, *vbB, *Pf, *Wf, *mub, *part, *gl, *gh, *acc, *fin_f, *fin_r;
     const size_t per_ck = 20 * (size_t)N;          // f (19N) + rho (N)
-    const size_t per_tape = 4 * (size_t)N;          // rho (N) + u (3N)
+    // tape entry of step n: rho^n after the wall fill (N), u^n (3N), mu^n (N)
+    const size_t per_tape = 5 * (size_t)N;
     const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 * 2 + 3 + 3 + 1) + std::max<long long>(nb, nbg) + 2 * NXY + 4 + 20 * (size_t)N;
     double acc_h[2] = {0, 0};
 
     AHIP(hipSetDevice(c->p.device));
-    AHIP(hipMalloc((void **)&ckpt, nseg * per_ck * sizeof(double)));
-    AHIP(hipMalloc((void **)&tape, (C + 1) * per_tape * sizeof(double)));
-    AHIP(hipMalloc((void **)&buf, nbuf * sizeof(double)));
-    AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
     {
-        double *p = buf;
-        fbA = p; p += 19 * N; fbB = p; p += 19 * N;
-        rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
-        vbA = p; p += 3 * N; vbB = p; p += 3 * N;
-        Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
-        part = p; p += std::max<long long>(nb, nbg); gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
-        fin_f = p; p += 19 * N; fin_r = p; p += N;
-    }
-    AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
-    AHIP(hipMemsetAsync(buf, 0, nbuf * sizeof(double), s));
-
-    // ---- forward with checkpoints ----
-    for (int sg = 0; sg < nseg; ++sg) {
-        double *ck = ckpt + sg * per_ck;
-        AHIP(hipMemcpyAsync(ck, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
-        AHIP(hipMemcpyAsync(ck + 19 * N, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-        const int L = std::min(C, d->nsteps - sg * C);
-        for (int k = 0; k < L; ++k) AHIP(launch_step(c, false));
-    }
-    AHIP(hipMemcpyAsync(fin_f, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    AHIP(hipMemcpyAsync(fin_r, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_adj_J, dim3(nb), dim3(TPB), 0, s, P, (const double *)c->rho, (const double *)tgt, rb, part);
-    AHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nb, acc);
-    AHIP(hipGetLastError());
-
-    // ---- reverse, segment by segment ----
-    for (int sg = nseg - 1; sg >= 0; --sg) {
-        const double *ck = ckpt + sg * per_ck;
-        AHIP(hipMemcpyAsync(c->f[c->cur], ck, 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
-        AHIP(hipMemcpyAsync(c->rho, ck + 19 * N, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-        const int L = std::min(C, d->nsteps - sg * C);
-        for (int k = 0; k < L; ++k) {
-            double *tr = tape + k * per_tape;
-            AHIP(launch_wall(c));
-            AHIP(hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-            AHIP(launch_mu(c));
-            AHIP(launch_collide(c, false, tr + N));
-            AHIP(launch_macro(c));
+        // trailing steps taped during the first forward run (no recompute); the
+        // rest is covered by checkpoints every C steps and recomputed per segment
+        long long budget = d->tape_budget_bytes;
+        if (budget < 0) {
+            size_t fr = 0, tot = 0;
+            AHIP(hipMemGetInfo(&fr, &tot));
+            budget = (long long)(fr / 2);
         }
+        const long long per_step = (long long)(per_tape * sizeof(double));
+        long long T = budget > per_step ? budget / per_step - 1 : 0;   // one entry for the final state
+        T = std::min<long long>(T, d->nsteps);
+        const int ntaped = (int)T;
+        const int n0 = d->nsteps - ntaped;              // steps covered by checkpoints
+        const int nseg = n0 > 0 ? (n0 + C - 1) / C : 0;
+        const int tape_len = std::max(ntaped, n0 > 0 ? C : 0) + 1;
+
+        AHIP(hipMalloc((void **)&ckpt, (size_t)std::max(nseg, 1) * per_ck * sizeof(double)));
+        AHIP(hipMalloc((void **)&tape, (size_t)tape_len * per_tape * sizeof(double)));
+        AHIP(hipMalloc((void **)&buf, nbuf * sizeof(double)));
+        AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
         {
-            double *tr = tape + L * per_tape;
-            AHIP(hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-            AHIP(launch_velocity(c, tr + N));
+            double *p = buf;
+            fbA = p; p += 19 * N; fbB = p; p += 19 * N;
+            rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
+            vbA = p; p += 3 * N; vbB = p; p += 3 * N;
+            Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
+            part = p; p += std::max<long long>(nb, nbg); gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
+            fin_f = p; p += 19 * N; fin_r = p; p += N;
         }
-        for (int k = L - 1; k >= 0; --k) {
-            const double *t0 = tape + k * per_tape, *t1 = tape + (k + 1) * per_tape;
-            // mu^n of the step (solid mu = wall field) from the taped density
-            AHIP(hipMemcpyAsync(c->rho, t0, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-            AHIP(launch_wall(c));
-            AHIP(launch_mu(c));
-            hipLaunchKernelGGL(k_adj_collide, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA, (const double *)rb,
-                               (const double *)vbA, t1, t1 + N, t0, t0 + N, (const double *)c->mu, fbB, vbB, rbl, Pf, Wf);
-            AHIP(hipGetLastError());
-            hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
-                               (const double *)rbl, mub, rbn);
-            AHIP(hipGetLastError());
-            hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, part);
-            AHIP(hipGetLastError());
-            hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nbg, acc + 1);
-            AHIP(hipGetLastError());
-            if (passes > 0) {
-                hipLaunchKernelGGL(k_adj_wall_g, dim3(nbc), dim3(TPB), 0, s, P, (const double *)mub, gl, gh);
-                AHIP(hipGetLastError());
-                for (int pass = passes - 1; pass >= 0; --pass) {
-                    hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass);
-                    AHIP(hipGetLastError());
+        AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
+        AHIP(hipMemsetAsync(buf, 0, nbuf * sizeof(double), s));
+
+        // one forward step that records its tape entry tr (rho after the wall
+        // fill, mu, u), and the entry of the final state (rho, u)
+        auto taped_step = [&](double *tr) -> hipError_t {
+            hipError_t e;
+            if ((e = launch_wall(c)) != hipSuccess) return e;
+            if ((e = hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+            if ((e = launch_mu(c)) != hipSuccess) return e;
+            if ((e = hipMemcpyAsync(tr + 4 * N, c->mu, N * sizeof(double), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+            if ((e = launch_collide(c, false, tr + N)) != hipSuccess) return e;
+            return launch_macro(c);
+        };
+        auto final_entry = [&](double *tr) -> hipError_t {
+            hipError_t e = hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s);
+            return e != hipSuccess ? e : launch_velocity(c, tr + N);
+        };
+        // reverse sweep over L taped steps (entries 0..L)
+        auto reverse = [&](int L) -> hipError_t {
+            for (int k = L - 1; k >= 0; --k) {
+                const double *t0 = tape + k * per_tape, *t1 = tape + (k + 1) * per_tape;
+                hipLaunchKernelGGL(k_adj_collide, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA, (const double *)rb,
+                                   (const double *)vbA, t1, t1 + N, t0, t0 + N, t0 + 4 * N, fbB, vbB, rbl, Pf, Wf);
+                hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
+                                   (const double *)rbl, mub, rbn);
+                hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, part);
+                hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nbg, acc + 1);
+                if (passes > 0) {
+                    hipLaunchKernelGGL(k_adj_wall_g, dim3(nbc), dim3(TPB), 0, s, P, (const double *)mub, gl, gh);
+                    for (int pass = passes - 1; pass >= 0; --pass)
+                        hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass);
+                    hipLaunchKernelGGL(k_adj_zero_solid, dim3(nb), dim3(TPB), 0, s, P, rbn);
                 }
-                hipLaunchKernelGGL(k_adj_zero_solid, dim3(nb), dim3(TPB), 0, s, P, rbn);
-                AHIP(hipGetLastError());
+                const hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+                std::swap(fbA, fbB);
+                std::swap(vbA, vbB);
+                std::swap(rb, rbn);
             }
-            std::swap(fbA, fbB);
-            std::swap(vbA, vbB);
-            std::swap(rb, rbn);
+            return hipSuccess;
+        };
+
+        // ---- forward: checkpoints over the first n0 steps, then taped steps ----
+        for (int sg = 0; sg < nseg; ++sg) {
+            double *ck = ckpt + sg * per_ck;
+            AHIP(hipMemcpyAsync(ck, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            AHIP(hipMemcpyAsync(ck + 19 * N, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            const int L = std::min(C, n0 - sg * C);
+            for (int k = 0; k < L; ++k) AHIP(launch_step(c, false));
+        }
+        for (int k = 0; k < ntaped; ++k) AHIP(taped_step(tape + k * per_tape));
+        if (ntaped > 0) AHIP(final_entry(tape + ntaped * per_tape));
+        AHIP(hipMemcpyAsync(fin_f, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+        AHIP(hipMemcpyAsync(fin_r, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_adj_J, dim3(nb), dim3(TPB), 0, s, P, (const double *)c->rho, (const double *)tgt, rb, part);
+        AHIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nb, acc);
+        AHIP(hipGetLastError());
+
+        // ---- reverse: the taped steps, then the checkpoint segments ----
+        if (ntaped > 0) AHIP(reverse(ntaped));
+        for (int sg = nseg - 1; sg >= 0; --sg) {
+            const double *ck = ckpt + sg * per_ck;
+            AHIP(hipMemcpyAsync(c->f[c->cur], ck, 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            AHIP(hipMemcpyAsync(c->rho, ck + 19 * N, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+            const int L = std::min(C, n0 - sg * C);
+            for (int k = 0; k < L; ++k) AHIP(taped_step(tape + k * per_tape));
+            AHIP(final_entry(tape + L * per_tape));
+            AHIP(reverse(L));
         }
     }
     AHIP(hipMemcpyAsync(acc_h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));

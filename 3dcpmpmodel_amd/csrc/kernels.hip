@@ -694,11 +694,15 @@ __global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *_
         f[q] = pop_load<TF, TC>(fp, pull_index(nb, (unsigned)P.N, q), q, P.rho0);
         s += f[q];
     }
-    TC jx, jy, jz;
-    momentum(f, jx, jy, jz);
-    out[I] = (double)(jx / s);
-    out[P.N + I] = (double)(jy / s);
-    out[2 * P.N + I] = (double)(jz / s);
+    // the same arithmetic as the velocity of collide(): moment rows 3, 5, 7 times
+    // the reciprocal density (the adjoint's tape entries then agree bit for bit
+    // whichever kernel recorded them)
+    TC m[Q];
+    moments(f, m);
+    const TC inv = TC(1) / s;
+    out[I] = (double)(m[3] * inv);
+    out[P.N + I] = (double)(m[5] * inv);
+    out[2 * P.N + I] = (double)(m[7] * inv);
 }
 
 // force field F = -rho grad(mu) + grad(rho)/3 (NonidealForce) into 3N doubles

@@ -113,8 +113,10 @@ def accuracy(cpmp, steps):
 
 
 def adjoint_cost(cpmp, nsteps=2000, every=100):
-    """adjoint (forward with checkpoints + recompute + reverse) time over forward
-    time, 96x96x64 sessile PR droplet, Tr = 0.75, FP64."""
+    """adjoint time over forward time, 96x96x64 sessile PR droplet, Tr = 0.75,
+    FP64, two schedules: checkpoints every `every` steps with every segment
+    recomputed, and the tape of all steps held in device memory (40 B per node
+    and step: 47 GB for 2000 steps) so that nothing is recomputed."""
     import numpy as np
     nx, ny, nz = 96, 96, 64
     p = cpmp.make_params(nx, ny, nz, eos="PRW", tr=0.75)
@@ -133,13 +135,17 @@ def adjoint_cost(cpmp, nsteps=2000, every=100):
     t0 = time.perf_counter()
     g.step(nsteps)
     t_fwd = time.perf_counter() - t0
-    g.init_from_rho(rho0.ravel())
-    t0 = time.perf_counter()
-    J, glo, ghi, gka = g.adjoint(nsteps, target, checkpoint_every=every)
-    t_adj = time.perf_counter() - t0
+    res = {}
+    for name, budget in (("recompute", 0), ("taped", -1)):
+        g.init_from_rho(rho0.ravel())
+        t0 = time.perf_counter()
+        J, glo, ghi, gka = g.adjoint(nsteps, target, checkpoint_every=every, tape_budget_bytes=budget)
+        res[name] = (time.perf_counter() - t0, J)
     g.close()
     return {"config": f"{nx}x{ny}x{nz} PRW Tr=0.75, {nsteps} steps, checkpoints every {every}",
-            "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd, "J": J}
+            "forward_s": t_fwd, "adjoint_s": res["recompute"][0], "adjoint_over_forward": res["recompute"][0] / t_fwd,
+            "adjoint_taped_s": res["taped"][0], "adjoint_taped_over_forward": res["taped"][0] / t_fwd,
+            "J": res["recompute"][1], "J_taped": res["taped"][1]}
 
 
 def cpu_baseline(n, steps):
@@ -207,7 +213,8 @@ def main():
         log(f"[bench] contact angle err {acc['contact_angle_err_deg']}, coexistence {acc['coexistence_rel_err']}")
         log("[bench] adjoint cost ...")
         adj = adjoint_cost(cpmp)
-        log(f"[bench] adjoint/forward = {adj['adjoint_over_forward']:.2f}")
+        log(f"[bench] adjoint/forward = {adj['adjoint_over_forward']:.2f} (checkpoints + recompute), "
+            f"{adj['adjoint_taped_over_forward']:.2f} (tape in device memory)")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
@@ -240,6 +247,7 @@ def main():
         "coexistence_rel_err": acc["coexistence_rel_err"] if acc else None,
         "laplace_rel_err": acc["laplace_rel_err"] if acc else None,
         "adjoint_over_forward": round(adj["adjoint_over_forward"], 3) if adj else None,
+        "adjoint_taped_over_forward": round(adj["adjoint_taped_over_forward"], 3) if adj else None,
         "accuracy": acc,
         "adjoint": adj,
         "cpu_baseline": cpu,

@@ -190,7 +190,13 @@ def test_adjoint_96x96x64_configuration(cpmp):
     t0 = time.perf_counter()
     J, glo, ghi, gka = lat.adjoint(nsteps, target, checkpoint_every=100)
     t_adj = time.perf_counter() - t0
-    print(f"[c4] J={J:.6e} |g|={np.linalg.norm(glo):.4e} adjoint/forward time = {t_adj / t_fwd:.2f}", flush=True)
+    lat.init_from_rho(rho0)
+    t0 = time.perf_counter()
+    Jt, glot, _, gkat = lat.adjoint(nsteps, target, checkpoint_every=100, tape_budget_bytes=-1)
+    t_tape = time.perf_counter() - t0
+    print(f"[c4] J={J:.6e} |g|={np.linalg.norm(glo):.4e} adjoint/forward time = {t_adj / t_fwd:.2f} "
+          f"(checkpoints + recompute), {t_tape / t_fwd:.2f} (tape in device memory)", flush=True)
+    assert Jt == J and np.array_equal(glot, glo) and gkat == gka
     assert np.isfinite(J) and J > 0
     rng = np.random.default_rng(6)
     h = 1e-6
@@ -202,3 +208,26 @@ def test_adjoint_96x96x64_configuration(cpmp):
         worst = max(worst, abs(ad - fd) / abs(fd))
         print(f"[c4-fd] dir {i}: adjoint {ad:.8e} fd {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
     assert worst < 1e-4
+
+
+def test_adjoint_tape_budget_schedules_agree(cpmp):
+    """taping all steps, taping the trailing 13 of 40 (the rest recomputed from
+    checkpoints every 6 steps) and recomputing everything run the same FP64
+    forward steps: identical J and gradients"""
+    dims = (16, 12, 20)
+    kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims)
+    n = dims[0] * dims[1] * dims[2]
+    per_step = 5 * n * 8
+    res = []
+    for budget in (0, (13 + 1) * per_step, -1):
+        g = cpmp.Lattice(cpmp.make_params(*dims, **kw))
+        g.set_wall_mu(mu_lo, mu_hi)
+        g.init_from_rho(rho0)
+        res.append(g.adjoint(40, target, checkpoint_every=6, tape_budget_bytes=budget))
+        assert g.step_count == 40
+        g.close()
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        assert np.array_equal(r[1], res[0][1])
+        assert np.array_equal(r[2], res[0][2])
+        assert r[3] == res[0][3]

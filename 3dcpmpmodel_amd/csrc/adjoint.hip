@@ -59,10 +59,32 @@ __global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const double *
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// Cotangent of the moments of step n+1 in the form the pulled populations need:
+// fhat_q = fbar_q + rbar + ubar.(e_q - u)/rho = fbar_q + A + e_q.B with
+// A = rbar - B.u, B = ubar/rho, packed (A, Bx, By, Bz) per node.  Solid nodes get
+// zeros (their rbar is cleared as well).
+__global__ void __launch_bounds__(TPB) k_adj_prep(const DevParams P, double *__restrict__ rb, const double *__restrict__ vb,
+                                                  const double *__restrict__ rho1, const double *__restrict__ u1,
+                                                  double2 *__restrict__ g) {
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I, N = P.N;
+    if (is_solid(P, nd.z)) {
+        rb[I] = 0;
+        g[2 * I] = make_double2(0, 0);
+        g[2 * I + 1] = make_double2(0, 0);
+        return;
+    }
+    const double r = rho1[I];
+    const double bx = vb[I] / r, by = vb[N + I] / r, bz = vb[2 * N + I] / r;
+    const double a = rb[I] - (bx * u1[I] + by * u1[N + I] + bz * u1[2 * N + I]);
+    g[2 * I] = make_double2(a, bx);
+    g[2 * I + 1] = make_double2(by, bz);
+}
+
 // reverse of moments + streaming + collision at each fluid node
 __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const double *__restrict__ fb_in,
-                                                     const double *__restrict__ rb_in, const double *__restrict__ vb_in,
-                                                     const double *__restrict__ rho1, const double *__restrict__ u1,
+                                                     const double2 *__restrict__ g,
                                                      const double *__restrict__ rho0, const double *__restrict__ u0,
                                                      const double *__restrict__ mu0, double *__restrict__ fb_out,
                                                      double *__restrict__ vb_out, double *__restrict__ rb_loc,
@@ -90,9 +112,8 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
         const long long y = bounce ? I : ((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz;
         const int qq = bounce ? opp(q) : q;
         const double ex = bounce ? -cx(q) : cx(q), ey = bounce ? -cy(q) : cy(q), ez = bounce ? -cz(q) : cz(q);
-        D[q] = fb_in[(long long)qq * N + y] + rb_in[y]
-               + (vb_in[y] * (ex - u1[y]) + vb_in[N + y] * (ey - u1[N + y]) + vb_in[2 * N + y] * (ez - u1[2 * N + y]))
-                     / rho1[y];
+        const double2 g0 = g[2 * y], g1 = g[2 * y + 1];
+        D[q] = fb_in[(long long)qq * N + y] + g0.x + (ex * g0.y + ey * g1.x + ez * g1.y);
     }
     double m[Q], mb[Q], b[Q];
     moments(D, m);
@@ -228,11 +249,6 @@ __global__ void __launch_bounds__(TPB) k_adj_wall_pass(const DevParams P, double
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_adj_zero_solid(const DevParams P, double *__restrict__ rb) {
-    const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (I < P.N && is_solid(P, (int)((unsigned long long)I % (unsigned)P.nz))) rb[I] = 0;
-}
-
 // acc[0] += sum(part[0..nb)) in a fixed order
 __global__ void __launch_bounds__(TPB) k_sum_into(const double *__restrict__ part, long long nb, double *__restrict__ acc) {
     double s = 0;
@@ -267,11 +283,11 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const int passes = std::max(P.lo, P.hi);
     const int64_t step0 = c->step;
     double *ckpt = nullptr, *tape = nullptr, *buf = nullptr, *tgt = nullptr;
-    double *fbA, *fbB, *rb, *rbn, *rbl, *vbA, *vbB, *Pf, *Wf, *mub, *part, *gl, *gh, *acc, *fin_f, *fin_r;
+    double *fbA, *fbB, *rb, *rbn, *rbl, *vbA, *vbB, *Pf, *Wf, *mub, *part, *gl, *gh, *acc, *fin_f, *fin_r, *g4;
     const size_t per_ck = 20 * (size_t)N;          // f (19N) + rho (N)
     // tape entry of step n: rho^n after the wall fill (N), u^n (3N), mu^n (N)
     const size_t per_tape = 5 * (size_t)N;
-    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 * 2 + 3 + 3 + 1) + std::max<long long>(nb, nbg) + 2 * NXY + 4 + 20 * (size_t)N;
+    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 * 2 + 3 + 3 + 1 + 4) + std::max<long long>(nb, nbg) + 2 * NXY + 4 + 20 * (size_t)N;
     double acc_h[2] = {0, 0};
 
     AHIP(hipSetDevice(c->p.device));
@@ -298,6 +314,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
         {
             double *p = buf;
+            g4 = p; p += 4 * N;   // first: the double2 loads need 16-byte alignment
             fbA = p; p += 19 * N; fbB = p; p += 19 * N;
             rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
             vbA = p; p += 3 * N; vbB = p; p += 3 * N;
@@ -327,8 +344,10 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         auto reverse = [&](int L) -> hipError_t {
             for (int k = L - 1; k >= 0; --k) {
                 const double *t0 = tape + k * per_tape, *t1 = tape + (k + 1) * per_tape;
-                hipLaunchKernelGGL(k_adj_collide, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA, (const double *)rb,
-                                   (const double *)vbA, t1, t1 + N, t0, t0 + N, t0 + 4 * N, fbB, vbB, rbl, Pf, Wf);
+                hipLaunchKernelGGL(k_adj_prep, node_grid(P), dim3(TPB), 0, s, P, rb, (const double *)vbA, t1, t1 + N,
+                                   (double2 *)g4);
+                hipLaunchKernelGGL(k_adj_collide, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA,
+                                   (const double2 *)g4, t0, t0 + N, t0 + 4 * N, fbB, vbB, rbl, Pf, Wf);
                 hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
                                    (const double *)rbl, mub, rbn);
                 hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, part);
@@ -337,7 +356,6 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
                     hipLaunchKernelGGL(k_adj_wall_g, dim3(nbc), dim3(TPB), 0, s, P, (const double *)mub, gl, gh);
                     for (int pass = passes - 1; pass >= 0; --pass)
                         hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass);
-                    hipLaunchKernelGGL(k_adj_zero_solid, dim3(nb), dim3(TPB), 0, s, P, rbn);
                 }
                 const hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;

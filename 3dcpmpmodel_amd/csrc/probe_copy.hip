@@ -10,6 +10,11 @@
 //           takes one contiguous run of z-fastest blocks (shared partial lines
 //           of the shifted reads stay in one L2)
 //   sumpx : sum over the 19 pulled values (the density pass), remapped grid
+//   flat  : copy on a 1-D grid, one population per loop pass (contiguous 1 KB per
+//           workgroup and population); flatnt: the same with non-temporal stores
+//   marchr: the access pattern of the march collide without its arithmetic: 8 x 32
+//           (y, z) tiles marching 32 x planes, pull reads, non-temporal stores,
+//           reference node order; marcht: the same on tile-contiguous sites
 // Prints one line per pattern: ms per launch and GB/s of the known traffic.
 //   probe_copy [reps]
 #include <hip/hip_runtime.h>
@@ -69,6 +74,48 @@ __global__ void __launch_bounds__(256) k_pull_xcd(const float *__restrict__ in, 
     }
 }
 
+template <bool NT>
+__global__ void __launch_bounds__(256) k_flat(const float *__restrict__ in, float *__restrict__ out) {
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+        const float v = in[(unsigned)q * N + i];
+        if (NT) __builtin_nontemporal_store(v, &out[(unsigned)q * N + i]);
+        else out[(unsigned)q * N + i] = v;
+    }
+}
+
+// node site: reference order, or tile-contiguous 8 x 32 tiles within each x plane
+template <bool TILED>
+__device__ __forceinline__ unsigned site(int x, int y, int z) {
+    if (!TILED) return ((unsigned)x * NY + y) * NZ + z;
+    return (unsigned)x * NY * NZ + (unsigned)(((y / 8) * (NZ / 32) + z / 32) * 256 + (y % 8) * 32 + z % 32);
+}
+template <bool TILED>
+__global__ void __launch_bounds__(256) k_march(const float *__restrict__ in, float *__restrict__ out) {
+    constexpr int ntz = NZ / 32, nty = NY / 8, XC = 32;
+    int t = blockIdx.x;
+    const int tzi = t % ntz;
+    t /= ntz;
+    const int tyi = t % nty, ch = t / nty;
+    const int y = tyi * 8 + threadIdx.x / 32, z = tzi * 32 + threadIdx.x % 32;
+    const int ym = (y + NY - 1) % NY, yp = (y + 1) % NY, zm = (z + NZ - 1) % NZ, zp = (z + 1) % NZ;
+    for (int x = ch * XC; x < ch * XC + XC; ++x) {
+        const int xm = (x + NX - 1) % NX, xp = (x + 1) % NX;
+        float v[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) {
+            const int sx = cxq(q) > 0 ? xm : cxq(q) < 0 ? xp : x;
+            const int sy = cyq(q) > 0 ? ym : cyq(q) < 0 ? yp : y;
+            const int sz = czq(q) > 0 ? zm : czq(q) < 0 ? zp : z;
+            v[q] = in[(unsigned)q * N + site<TILED>(sx, sy, sz)];
+        }
+        const unsigned I = site<TILED>(x, y, z);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) __builtin_nontemporal_store(v[q], &out[(unsigned)q * N + I]);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_sum19(const float *__restrict__ in, float *__restrict__ out) {
     const int z = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6), x = blockIdx.z;
     const unsigned I = ((unsigned)x * NY + y) * NZ + z;
@@ -93,9 +140,9 @@ int main(int argc, char **argv) {
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    const char *names[6] = {"copy", "push", "pull", "sum19", "pullx", "sumpx"};
-    const dim3 g1((NZ / 64) * (NY / 4) * NX);
-    for (int mode = 0; mode < 6; ++mode) {
+    const char *names[10] = {"copy", "push", "pull", "sum19", "pullx", "sumpx", "flat", "flatnt", "marchr", "marcht"};
+    const dim3 g1((NZ / 64) * (NY / 4) * NX), gf(N / 256), gm((NX / 32) * (NY / 8) * (NZ / 32));
+    for (int mode = 0; mode < 10; ++mode) {
         float ms = 0;
         for (int pass = 0; pass < 2; ++pass) {   // first pass warms up
             hipEventRecord(e0);
@@ -106,6 +153,10 @@ int main(int argc, char **argv) {
                 if (mode == 3) hipLaunchKernelGGL(k_sum19, g, t, 0, 0, a, b);
                 if (mode == 4) hipLaunchKernelGGL(k_pull_xcd<0>, g1, t, 0, 0, a, b);
                 if (mode == 5) hipLaunchKernelGGL(k_pull_xcd<1>, g1, t, 0, 0, a, b);
+                if (mode == 6) hipLaunchKernelGGL(k_flat<false>, gf, t, 0, 0, a, b);
+                if (mode == 7) hipLaunchKernelGGL(k_flat<true>, gf, t, 0, 0, a, b);
+                if (mode == 8) hipLaunchKernelGGL(k_march<false>, gm, t, 0, 0, a, b);
+                if (mode == 9) hipLaunchKernelGGL(k_march<true>, gm, t, 0, 0, a, b);
             }
             hipEventRecord(e1);
             hipEventSynchronize(e1);

@@ -112,6 +112,23 @@ def accuracy(cpmp, steps):
     return res
 
 
+def measured_traffic(kernel_key):
+    """Bytes per launch that left the L2 for one kernel of the 256^3 workload, from
+    the committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE passes calibrated with a
+    copy kernel of known size); None if the summary does not list the kernel."""
+    import re
+    path = os.path.join(ROOT, "profiles", "step_256cubed_rke_kernel_stats.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                m = re.search(r"traffic_bytes_per_launch (\S+) (\d+)", line)
+                if m and m.group(1) == kernel_key:
+                    return int(m.group(2))
+    except OSError:
+        pass
+    return None
+
+
 def adjoint_cost(cpmp, nsteps=2000, every=100):
     """adjoint time over forward time, 96x96x64 sessile PR droplet, Tr = 0.75,
     FP64, two schedules: checkpoints every `every` steps with every segment
@@ -203,8 +220,9 @@ def main():
     kc = main_form["kernel_ms"]["collide"]
     achieved = main_form["bytes_per_node"] * nodes / (kc * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None,
-                    kernel="k_collide (fp32)", bytes_per_node=main_form["bytes_per_node"])
+                    frac=round(achieved / PEAK_HBM_GBS, 4),
+                    traffic=measured_traffic("k_collide_march_fp32") if n == 256 else None,
+                    kernel="k_collide_march (fp32)", bytes_per_node=main_form["bytes_per_node"])
     step_bw = main_form["bytes_per_node"] * nodes / (main_form["ms_per_step"] * 1e-3) / 1e9
     acc = adj = None
     if not args.no_accuracy:

@@ -14,7 +14,9 @@
 //   mu      : mu = K^2 mu_eos(clamp rho) - Ka Lap(rho) (fluid), mu = mu_w (solid)
 //   wall    : solid rho = layered diagonal averages of the first fluid layer (symmetric)
 // The forward trajectory is recomputed segment by segment from population
-// checkpoints; within a segment the tape holds rho (after the wall fill) and u.
+// checkpoints; the tape holds rho (after the wall fill), u, grad mu and grad rho of
+// every step of the segment.  Trailing steps that fit the tape budget are taped during the first
+// forward run and not recomputed.
 #include "internal.h"
 #include "lbm_math.h"
 #include "device_common.h"
@@ -59,35 +61,18 @@ __global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const double *
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// Cotangent of the moments of step n+1 in the form the pulled populations need:
-// fhat_q = fbar_q + rbar + ubar.(e_q - u)/rho = fbar_q + A + e_q.B with
-// A = rbar - B.u, B = ubar/rho, packed (A, Bx, By, Bz) per node.  Solid nodes get
-// zeros (their rbar is cleared as well).
-__global__ void __launch_bounds__(TPB) k_adj_prep(const DevParams P, double *__restrict__ rb, const double *__restrict__ vb,
-                                                  const double *__restrict__ rho1, const double *__restrict__ u1,
-                                                  double2 *__restrict__ g) {
-    const NodeIdx nd = node_from_grid(P);
-    if (!nd.ok) return;
-    const long long I = nd.I, N = P.N;
-    if (is_solid(P, nd.z)) {
-        rb[I] = 0;
-        g[2 * I] = make_double2(0, 0);
-        g[2 * I + 1] = make_double2(0, 0);
-        return;
-    }
-    const double r = rho1[I];
-    const double bx = vb[I] / r, by = vb[N + I] / r, bz = vb[2 * N + I] / r;
-    const double a = rb[I] - (bx * u1[I] + by * u1[N + I] + bz * u1[2 * N + I]);
-    g[2 * I] = make_double2(a, bx);
-    g[2 * I + 1] = make_double2(by, bz);
-}
-
-// reverse of moments + streaming + collision at each fluid node
-__global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const double *__restrict__ fb_in,
-                                                     const double2 *__restrict__ g,
+// Reverse of moments + streaming + collision at each fluid node of step n.  The
+// cotangent arriving along q from the node y that pulls it is
+//   fhat_q(y) = fbar_q(y) + rbar(y) + ubar(y).(c_q - u(y))/rho(y)
+// with (fbar, ubar) of step n+1 at y and rho, u of the state y pulled; the part
+// without rbar is local to y and was written by the previous reverse step as fh_in
+// (fbar_q + ubar.(c_q - u)/rho), so each pulled value costs two loads.  This kernel
+// writes the same folded form for step n: fh_out_q = fbar_q + ubar.(c_q - u^n)/rho^n.
+__global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const double *__restrict__ fh_in,
+                                                     const double *__restrict__ rb_in,
                                                      const double *__restrict__ rho0, const double *__restrict__ u0,
-                                                     const double *__restrict__ mu0, double *__restrict__ fb_out,
-                                                     double *__restrict__ vb_out, double *__restrict__ rb_loc,
+                                                     const double *__restrict__ grads, double *__restrict__ fh_out,
+                                                     double *__restrict__ rb_loc,
                                                      double *__restrict__ Pf, double *__restrict__ Wf) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
@@ -96,9 +81,9 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
     const Coord c{nd.x, nd.y, nd.z};
     if (is_solid(P, c.z)) {
         rb_loc[I] = 0;
-        for (int a = 0; a < 3; ++a) { vb_out[a * N + I] = 0; Pf[a * N + I] = 0; Wf[a * N + I] = 0; }
+        for (int a = 0; a < 3; ++a) { Pf[a * N + I] = 0; Wf[a * N + I] = 0; }
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fb_out[(long long)q * N + I] = 0;
+        for (int q = 0; q < Q; ++q) fh_out[(long long)q * N + I] = 0;
         return;
     }
     int xs[7], ys[7], zs[7];
@@ -110,10 +95,8 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
         const int tz = zs[3 + cz(q)];
         const bool bounce = is_solid(P, tz);
         const long long y = bounce ? I : ((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz;
-        const int qq = bounce ? opp(q) : q;
-        const double ex = bounce ? -cx(q) : cx(q), ey = bounce ? -cy(q) : cy(q), ez = bounce ? -cz(q) : cz(q);
-        const double2 g0 = g[2 * y], g1 = g[2 * y + 1];
-        D[q] = fb_in[(long long)qq * N + y] + g0.x + (ex * g0.y + ey * g1.x + ez * g1.y);
+        const int qq = bounce ? opp(q) : q;   // bounced back: pulled by this node as opp(q), c_opp(q) = -c_q
+        D[q] = fh_in[(long long)qq * N + y] + rb_in[y];
     }
     double m[Q], mb[Q], b[Q];
     moments(D, m);
@@ -124,14 +107,11 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
     }
     double fbar[Q];
     inverse_scaled(b, fbar);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) fb_out[(long long)q * N + I] = fbar[q];
 
     const double rho = rho0[I], ux = u0[I], uy = u0[N + I], uz = u0[2 * N + I];
-    const Stencil<double, double> st{P, xs, ys, zs};
-    double gmx, gmy, gmz, grx, gry, grz;
-    st.grad(mu0, gmx, gmy, gmz);
-    st.grad(rho0, grx, gry, grz);
+    // grad mu^n and grad rho^n as the forward collide evaluated them (taped)
+    const double gmx = grads[I], gmy = grads[N + I], gmz = grads[2 * N + I];
+    const double grx = grads[3 * N + I], gry = grads[4 * N + I], grz = grads[5 * N + I];
     const double Fx = -gmx * rho + grx / 3, Fy = -gmy * rho + gry / 3, Fz = -gmz * rho + grz / 3;
     const double ex = ux + Fx / rho, ey = uy + Fy / rho, ez = uz + Fz / rho;
     double wu[16];
@@ -143,9 +123,13 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
     const double Fbx = dex / rho, Fby = dey / rho, Fbz = dez / rho;
     double rbar = dru + dre - (dex * Fx + dey * Fy + dez * Fz) / (rho * rho);
     rbar -= gmx * Fbx + gmy * Fby + gmz * Fbz;
-    vb_out[I] = dux + dex;
-    vb_out[N + I] = duy + dey;
-    vb_out[2 * N + I] = duz + dez;
+    {
+        const double ubx = (dux + dex) / rho, uby = (duy + dey) / rho, ubz = (duz + dez) / rho;
+        const double ubu = ubx * ux + uby * uy + ubz * uz;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            fh_out[(long long)q * N + I] = fbar[q] + (cx(q) * ubx + cy(q) * uby + cz(q) * ubz - ubu);
+    }
     double wx = Fbx / 3, wy = Fby / 3, wz = Fbz / 3;
     if (P.qp_on) {
         const double K1 = P.K1, c1 = -19 * (P.K1 + 3 * P.K2);
@@ -212,27 +196,28 @@ __global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const double 
         }
         rb_new[I] += acc;
     }
+    // each block adds its partial to its own slot, step after step (one writer per
+    // slot and launch: a fixed order); the slots are summed once at the end
     const double s = block_sum(kap);
-    if (threadIdx.x == 0) kapart[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = s;
+    if (threadIdx.x == 0) kapart[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] += s;
 }
 
-// wall-field gradients: solid mu = mu_w of the column
-__global__ void __launch_bounds__(TPB) k_adj_wall_g(const DevParams P, const double *__restrict__ mub,
-                                                    double *__restrict__ glo, double *__restrict__ ghi) {
+// transposed wall fill, one layer pass (run deepest pass first).  The deepest
+// pass also adds the wall-field gradients (solid mu = mu_w of the column) when
+// mub is given.
+__global__ void __launch_bounds__(TPB) k_adj_wall_pass(const DevParams P, double *__restrict__ rb, int pass,
+                                                       const double *__restrict__ mub, double *__restrict__ glo,
+                                                       double *__restrict__ ghi) {
     const long long col = (long long)blockIdx.x * TPB + threadIdx.x;
     if (col >= (long long)P.nx * P.ny) return;
-    const double *m = mub + col * P.nz;
-    double a = 0, b = 0;
-    for (int k = 0; k < P.lo; ++k) a += m[k];
-    for (int k = P.nz - P.hi; k < P.nz; ++k) b += m[k];
-    glo[col] += a;
-    ghi[col] += b;
-}
-
-// transposed wall fill, one layer pass (run deepest pass first)
-__global__ void __launch_bounds__(TPB) k_adj_wall_pass(const DevParams P, double *__restrict__ rb, int pass) {
-    const long long col = (long long)blockIdx.x * TPB + threadIdx.x;
-    if (col >= (long long)P.nx * P.ny) return;
+    if (mub) {
+        const double *m = mub + col * P.nz;
+        double a = 0, b = 0;
+        for (int k = 0; k < P.lo; ++k) a += m[k];
+        for (int k = P.nz - P.hi; k < P.nz; ++k) b += m[k];
+        glo[col] += a;
+        ghi[col] += b;
+    }
     const int x = (int)(col / P.ny), y = (int)(col % P.ny);
     const int x1 = wrapc(x - 1, P.nx), x2 = wrapc(x + 1, P.nx), y1 = wrapc(y - 1, P.ny), y2 = wrapc(y + 1, P.ny);
     auto at = [&](int xx, int yy, int k) { return ((long long)xx * P.ny + yy) * P.nz + k; };
@@ -283,11 +268,12 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const int passes = std::max(P.lo, P.hi);
     const int64_t step0 = c->step;
     double *ckpt = nullptr, *tape = nullptr, *buf = nullptr, *tgt = nullptr;
-    double *fbA, *fbB, *rb, *rbn, *rbl, *vbA, *vbB, *Pf, *Wf, *mub, *part, *gl, *gh, *acc, *fin_f, *fin_r, *g4;
+    double *fbA, *fbB, *rb, *rbn, *rbl, *Pf, *Wf, *mub, *part, *kpart, *gl, *gh, *acc, *fin_f, *fin_r;
     const size_t per_ck = 20 * (size_t)N;          // f (19N) + rho (N)
-    // tape entry of step n: rho^n after the wall fill (N), u^n (3N), mu^n (N)
-    const size_t per_tape = 5 * (size_t)N;
-    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 * 2 + 3 + 3 + 1 + 4) + std::max<long long>(nb, nbg) + 2 * NXY + 4 + 20 * (size_t)N;
+    // tape entry of step n: rho^n after the wall fill (N), then u^n (3N), grad mu^n
+    // (3N) and grad rho^n (3N) as the forward collide evaluated them
+    const size_t per_tape = 10 * (size_t)N;
+    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 + 3 + 1) + std::max<long long>(nb, nbg) + nbg + 2 * NXY + 4 + 20 * (size_t)N;
     double acc_h[2] = {0, 0};
 
     AHIP(hipSetDevice(c->p.device));
@@ -301,12 +287,12 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             budget = (long long)(fr / 2);
         }
         const long long per_step = (long long)(per_tape * sizeof(double));
-        long long T = budget > per_step ? budget / per_step - 1 : 0;   // one entry for the final state
+        long long T = budget / per_step;
         T = std::min<long long>(T, d->nsteps);
         const int ntaped = (int)T;
         const int n0 = d->nsteps - ntaped;              // steps covered by checkpoints
         const int nseg = n0 > 0 ? (n0 + C - 1) / C : 0;
-        const int tape_len = std::max(ntaped, n0 > 0 ? C : 0) + 1;
+        const int tape_len = std::max(std::max(ntaped, n0 > 0 ? C : 0), 1);
 
         AHIP(hipMalloc((void **)&ckpt, (size_t)std::max(nseg, 1) * per_ck * sizeof(double)));
         AHIP(hipMalloc((void **)&tape, (size_t)tape_len * per_tape * sizeof(double)));
@@ -314,53 +300,40 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
         {
             double *p = buf;
-            g4 = p; p += 4 * N;   // first: the double2 loads need 16-byte alignment
             fbA = p; p += 19 * N; fbB = p; p += 19 * N;
             rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
-            vbA = p; p += 3 * N; vbB = p; p += 3 * N;
             Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
-            part = p; p += std::max<long long>(nb, nbg); gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
+            part = p; p += std::max<long long>(nb, nbg); kpart = p; p += nbg;
+            gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
             fin_f = p; p += 19 * N; fin_r = p; p += N;
         }
         AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
         AHIP(hipMemsetAsync(buf, 0, nbuf * sizeof(double), s));
 
-        // one forward step that records its tape entry tr (rho after the wall
-        // fill, mu, u), and the entry of the final state (rho, u)
+        // one forward step that records its tape entry tr
         auto taped_step = [&](double *tr) -> hipError_t {
             hipError_t e;
             if ((e = launch_wall(c)) != hipSuccess) return e;
             if ((e = hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
             if ((e = launch_mu(c)) != hipSuccess) return e;
-            if ((e = hipMemcpyAsync(tr + 4 * N, c->mu, N * sizeof(double), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
             if ((e = launch_collide(c, false, tr + N)) != hipSuccess) return e;
             return launch_macro(c);
         };
-        auto final_entry = [&](double *tr) -> hipError_t {
-            hipError_t e = hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s);
-            return e != hipSuccess ? e : launch_velocity(c, tr + N);
-        };
-        // reverse sweep over L taped steps (entries 0..L)
+        // reverse sweep over L taped steps (entries 0..L-1)
         auto reverse = [&](int L) -> hipError_t {
             for (int k = L - 1; k >= 0; --k) {
-                const double *t0 = tape + k * per_tape, *t1 = tape + (k + 1) * per_tape;
-                hipLaunchKernelGGL(k_adj_prep, node_grid(P), dim3(TPB), 0, s, P, rb, (const double *)vbA, t1, t1 + N,
-                                   (double2 *)g4);
+                const double *t0 = tape + k * per_tape;
                 hipLaunchKernelGGL(k_adj_collide, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA,
-                                   (const double2 *)g4, t0, t0 + N, t0 + 4 * N, fbB, vbB, rbl, Pf, Wf);
+                                   (const double *)rb, t0, t0 + N, t0 + 4 * N, fbB, rbl, Pf, Wf);
                 hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
                                    (const double *)rbl, mub, rbn);
-                hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, part);
-                hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nbg, acc + 1);
-                if (passes > 0) {
-                    hipLaunchKernelGGL(k_adj_wall_g, dim3(nbc), dim3(TPB), 0, s, P, (const double *)mub, gl, gh);
-                    for (int pass = passes - 1; pass >= 0; --pass)
-                        hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass);
-                }
+                hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, kpart);
+                for (int pass = passes - 1; pass >= 0; --pass)
+                    hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass,
+                                       pass == passes - 1 ? (const double *)mub : nullptr, gl, gh);
                 const hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 std::swap(fbA, fbB);
-                std::swap(vbA, vbB);
                 std::swap(rb, rbn);
             }
             return hipSuccess;
@@ -375,7 +348,6 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             for (int k = 0; k < L; ++k) AHIP(launch_step(c, false));
         }
         for (int k = 0; k < ntaped; ++k) AHIP(taped_step(tape + k * per_tape));
-        if (ntaped > 0) AHIP(final_entry(tape + ntaped * per_tape));
         AHIP(hipMemcpyAsync(fin_f, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
         AHIP(hipMemcpyAsync(fin_r, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_adj_J, dim3(nb), dim3(TPB), 0, s, P, (const double *)c->rho, (const double *)tgt, rb, part);
@@ -391,10 +363,11 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             AHIP(hipMemcpyAsync(c->rho, ck + 19 * N, N * sizeof(double), hipMemcpyDeviceToDevice, s));
             const int L = std::min(C, n0 - sg * C);
             for (int k = 0; k < L; ++k) AHIP(taped_step(tape + k * per_tape));
-            AHIP(final_entry(tape + L * per_tape));
             AHIP(reverse(L));
         }
     }
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)kpart, nbg, acc + 1);
+    AHIP(hipGetLastError());
     AHIP(hipMemcpyAsync(acc_h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     if (g_lo) AHIP(hipMemcpyAsync(g_lo, gl, NXY * sizeof(double), hipMemcpyDeviceToHost, s));
     if (g_hi) AHIP(hipMemcpyAsync(g_hi, gh, NXY * sizeof(double), hipMemcpyDeviceToHost, s));

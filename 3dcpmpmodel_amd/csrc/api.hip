@@ -57,6 +57,7 @@ void fill_devparams(cpmp_ctx *c) {
     d.A = p.A; d.B = p.B; d.T = p.T; d.K2s = p.K * p.K; d.Ka = p.Ka; d.K1 = p.K1; d.K2 = p.K2;
     for (int q = 0; q < 19; ++q) { d.ts[q] = p.Ts[q]; d.tsf[q] = (float)p.Ts[q]; }
     d.rho0 = p.rho0;
+    d.xc = march_chunk(p.nx, p.ny, p.nz);
 }
 
 int check_params(const cpmp_params *p) {
@@ -226,7 +227,7 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
     const char *fuse = std::getenv("CPMP_FUSE");
     size_t ring_elems = 0, chunk_elems = 0;
     if (c->form != CPMP_FORM_FP64 && !(fuse && std::strcmp(fuse, "0") == 0) &&
-        fuse_geometry(p->nx, p->ny, p->nz, &ring_elems, &chunk_elems)) {
+        fuse_geometry(p->nx, p->ny, p->nz, c->dp.xc, &ring_elems, &chunk_elems)) {
         if (hipMalloc(&c->rho_next, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "next rho");
         if (hipMalloc(&c->rbuf, ring_elems * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "tile rings");
         if (hipMalloc(&c->cbuf, chunk_elems * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "chunk planes");

@@ -24,6 +24,7 @@ struct DevParams {
     float tsf[19];              // ts in FP32 for the FP32 arithmetic forms
     double rho0;                // FP16 shift density
     int tiled, nty, ntz;        // tile-contiguous population sites (pop_site), tile counts in y, z
+    int xc;                     // x planes per workgroup of the march kernels
 };
 
 enum Kern { KW = CPMP_K_WALL, KM = CPMP_K_MU, KC = CPMP_K_COLLIDE, KR = CPMP_K_MACRO };
@@ -69,10 +70,12 @@ int set_error(int code, const char *fmt, ...);
 hipError_t launch_wall(cpmp_ctx *c);
 hipError_t launch_mu(cpmp_ctx *c);
 bool density_fused(const cpmp_ctx *c);
-bool fuse_geometry(int nx, int ny, int nz, size_t *ring_elems, size_t *chunk_elems);
+bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *chunk_elems);
+int march_chunk(int nx, int ny, int nz);
 constexpr int kTileY = 8, kTileZ = 32;   // in-plane tile of the march kernels and of tiled population storage
 // copy one population set between the device layout and the reference node order
 hipError_t launch_pop_relayout(cpmp_ctx *c, const void *src, void *dst, bool to_device_layout);
+// uout (when given): u, grad mu, grad rho of every node, 9 N doubles (adjoint tape)
 hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout = nullptr);
 hipError_t launch_macro(cpmp_ctx *c);
 hipError_t launch_step(cpmp_ctx *c, bool want_mv);

@@ -228,10 +228,10 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
     } else {
         collide(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
     }
-    if (uout) {   // velocity tape for the adjoint's recomputed forward
-        uout[I] = (double)u[0];
-        uout[N + I] = (double)u[1];
-        uout[2 * N + I] = (double)u[2];
+    if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
+        const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
+#pragma unroll
+        for (int a = 0; a < 9; ++a) uout[a * N + I] = (double)t9[a];
     }
     if (MV) {
         const TC h = TC(0.5) / r;
@@ -247,11 +247,11 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
 // ---------------------------------------------------------------------------
 // The same fused step as k_collide, organised for gfx950: a workgroup owns an
 // in-plane tile of MTY x MTZ nodes (z contiguous, 32 lanes per row) and marches
-// along x over MXC planes.  rho and mu of planes x-3..x+3 with an in-plane halo
+// along x over P.xc planes (march_chunk).  rho and mu of planes x-3..x+3 with an in-plane halo
 // of 3 sit in an LDS ring, so each value is fetched from L2/HBM about once per
 // tile instead of 54 times per stencil; the populations of plane x are loaded
 // (coalesced) before the ring update so their latency overlaps the barrier.
-constexpr int MTY = TILE_Y, MTZ = TILE_Z, MH = 3, MXC = 32;
+constexpr int MTY = TILE_Y, MTZ = TILE_Z, MH = 3, MXC = 32;   // MXC: longest chunk (P.xc <= MXC)
 constexpr int MLY = MTY + 2 * MH, MLZ = MTZ + 2 * MH, MLP = MLY * MLZ;
 
 template <class TS, class TC>
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
     const int tyi = t % nty;
     const int ch = t / nty;
     const int y0 = tyi * MTY, z0 = tzi * MTZ;
-    const int xa = ch * MXC, xb = min(xa + MXC, P.nx);
+    const int xa = ch * P.xc, xb = min(xa + P.xc, P.nx);
     const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
     const int y = y0 + ty, z = z0 + tz;
     const bool inr = (y < P.ny) && (z < P.nz);
@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
 // planes outside the chunk (xa-1, xb) to cbuf.  k_density_fix then adds the
 // rings of the neighbouring tiles and the partial planes of the neighbouring
 // chunks, in a fixed order, with no atomics to global memory.  Requires
-// ny % MTY == nz % MTZ == nx % MXC == 0 (fuse_geometry).
+// ny % MTY == nz % MTZ == nx % P.xc == 0 (fuse_geometry).
 constexpr int AY = MTY + 2, AZ = MTZ + 2, AP = AY * AZ, ARING = AP - MTY * MTZ;
 
 // accumulator position (ly, lz) of ring entry r: rows ly = 0 and ly = AY-1, then
@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
     const int tyi = t % nty;
     const int ch = t / nty;
     const int y0 = tyi * MTY, z0 = tzi * MTZ;
-    const int xa = ch * MXC, xb = min(xa + MXC, P.nx);
+    const int xa = ch * P.xc, xb = min(xa + P.xc, P.nx);
     const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
     const int y = y0 + ty, z = z0 + tz;
     const bool active = (y < P.ny) && (z < P.nz) && !is_solid(P, z);
@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
     auto flush = [&](int xp, TC own_v, TC ring_v) {
         if (xp >= xa && xp < xb) {
             if (active) rho_next[((long long)xp * P.ny + y) * P.nz + z] = (TS)own_v;
-            if (threadIdx.x < ARING) rbuf[((long long)tile * MXC + (xp - xa)) * ARING + threadIdx.x] = (TS)ring_v;
+            if (threadIdx.x < ARING) rbuf[(((long long)xp * nty + tyi) * ntz + tzi) * ARING + threadIdx.x] = (TS)ring_v;
         } else {
             TS *cb = cbuf + ((long long)tile * 2 + (xp < xa ? 0 : 1)) * AP;
             cb[own] = (TS)own_v;
@@ -516,10 +516,10 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
             } else {
                 collide(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
             }
-            if (uout) {
-                uout[I] = (double)u[0];
-                uout[N + I] = (double)u[1];
-                uout[2 * N + I] = (double)u[2];
+            if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
+                const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
+#pragma unroll
+                for (int a = 0; a < 9; ++a) uout[a * N + I] = (double)t9[a];
             }
             if (MV) {
                 const TC h = TC(0.5) / r;
@@ -581,11 +581,11 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
 // the partial plane xb / xa-1 of the previous / next chunk (its own tile's
 // entry and the neighbouring tiles' ring entries).  Tiles and chunks wrap
 // periodically, so a lattice one tile wide reads its own ring.
-// Border items of one tile: for each of its MXC planes the ETILE border positions
+// Border items of one tile: for each of its P.xc planes the ETILE border positions
 // (rows ty = 0 and MTY-1, then the z ends of the rows between), then on the first
 // and last plane the interior positions.
 constexpr int ETILE = 2 * MTZ + 2 * (MTY - 2), ITILE = (MTY - 2) * (MTZ - 2);
-constexpr int FIX_ITEMS = MXC * ETILE + 2 * ITILE;
+constexpr int FIX_ITEMS_MAX = MXC * ETILE + 2 * ITILE;
 
 template <class TS>
 __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__restrict__ rho, const TS *__restrict__ rbuf,
@@ -603,12 +603,14 @@ __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__re
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const int yy = (tyi + i / 3 - 1 + nty) % nty, zz = (tzi + i % 3 - 1 + ntz) % ntz;
-        rbase[i] = (((long long)ch * nty + yy) * ntz + zz) * MXC * ARING;
+        rbase[i] = ((long long)yy * ntz + zz) * ARING;   // in-plane part of the ring index
         pbase[i] = ((((long long)cp * nty + yy) * ntz + zz) * 2 + 1) * AP;
         nbase[i] = ((((long long)cn * nty + yy) * ntz + zz) * 2 + 0) * AP;
     }
     // every item of this thread at once (all loads in flight), then the updates
-    constexpr int PER = (FIX_ITEMS + TPB - 1) / TPB;
+    constexpr int PER = (FIX_ITEMS_MAX + TPB - 1) / TPB;
+    const int items = P.xc * ETILE + 2 * ITILE;
+    const long long plane_ring = (long long)nty * ntz * ARING;
     TS add[PER];
     long long at[PER];
 #pragma unroll
@@ -616,17 +618,17 @@ __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__re
         const int it = threadIdx.x + j * TPB;
         add[j] = TS(0);
         at[j] = -1;
-        if (it >= FIX_ITEMS) continue;
+        if (it >= items) continue;
         int k, ty, tz;
-        if (it < MXC * ETILE) {
+        if (it < P.xc * ETILE) {
             k = it / ETILE;
             const int e = it - k * ETILE;
             if (e < MTZ) { ty = 0; tz = e; }
             else if (e < 2 * MTZ) { ty = MTY - 1; tz = e - MTZ; }
             else { ty = 1 + ((e - 2 * MTZ) >> 1); tz = ((e - 2 * MTZ) & 1) ? MTZ - 1 : 0; }
         } else {
-            const int jj = it - MXC * ETILE;
-            k = jj < ITILE ? 0 : MXC - 1;
+            const int jj = it - P.xc * ETILE;
+            k = jj < ITILE ? 0 : P.xc - 1;
             const int e = jj < ITILE ? jj : jj - ITILE;
             ty = 1 + e / (MTZ - 2);
             tz = 1 + e % (MTZ - 2);
@@ -634,7 +636,7 @@ __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__re
         const int z = tzi * MTZ + tz;
         if (is_solid(P, z)) continue;
         const bool ey0 = ty == 0, ey1 = ty == MTY - 1, ez0 = tz == 0, ez1 = tz == MTZ - 1;
-        const bool lo = k == 0, hi = k == MXC - 1;
+        const bool lo = k == 0, hi = k == P.xc - 1;
         TS a = TS(0);
         // neighbouring tiles whose border ring holds this node (ring position ly,
         // lz): their ring entry of this plane, and on the chunk's first / last
@@ -642,7 +644,7 @@ __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__re
         auto from = [&](bool on, int dy, int dz, int ly, int lz) {
             if (!on) return;
             const int i = (dy + 1) * 3 + dz + 1;
-            a += rbuf[rbase[i] + k * ARING + ring_index(ly, lz)];
+            a += rbuf[(long long)(ch * P.xc + k) * plane_ring + rbase[i] + ring_index(ly, lz)];
             if (lo) a += cbuf[pbase[i] + ly * AZ + lz];
             if (hi) a += cbuf[nbase[i] + ly * AZ + lz];
         };
@@ -658,7 +660,7 @@ __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__re
         if (lo) a += cbuf[pbase[4] + (ty + 1) * AZ + tz + 1];
         if (hi) a += cbuf[nbase[4] + (ty + 1) * AZ + tz + 1];
         add[j] = a;
-        at[j] = ((long long)(ch * MXC + k) * P.ny + tyi * MTY + ty) * P.nz + z;
+        at[j] = ((long long)(ch * P.xc + k) * P.ny + tyi * MTY + ty) * P.nz + z;
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j)
@@ -952,12 +954,30 @@ static int mu_variant() {
 // FP32 storage forms with the march collide sum the density inside the collide
 // kernel when the tiles cover the lattice exactly (a context created with
 // CPMP_FUSE=0 in the environment keeps the separate density pass)
-bool fuse_geometry(int nx, int ny, int nz, size_t *ring_elems, size_t *chunk_elems) {
-    if (nx % MXC || ny % MTY || nz % MTZ) return false;
-    const size_t tiles = (size_t)(nx / MXC) * (ny / MTY) * (nz / MTZ);
-    *ring_elems = tiles * MXC * ARING;
-    *chunk_elems = tiles * 2 * AP;
+bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *chunk_elems) {
+    if (xc < 2 || xc > MXC || nx % xc || ny % MTY || nz % MTZ) return false;
+    const size_t tiles_yz = (size_t)(ny / MTY) * (nz / MTZ);
+    *ring_elems = (size_t)nx * tiles_yz * ARING;
+    *chunk_elems = (size_t)(nx / xc) * tiles_yz * 2 * AP;
     return true;
+}
+
+// Planes per workgroup of the march kernels: the longest chunk (32, 16, ..., 2)
+// that divides nx and still gives 2048 workgroups (8 per CU); small lattices get
+// short chunks so that they fill the GPU.  CPMP_XC overrides.
+int march_chunk(int nx, int ny, int nz) {
+    if (const char *e = std::getenv("CPMP_XC")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= MXC) return v;
+    }
+    const long long tiles = (long long)((ny + MTY - 1) / MTY) * ((nz + MTZ - 1) / MTZ);
+    for (int c = MXC; c >= 2; c /= 2)
+        if (nx % c == 0 && (nx / c) * tiles >= 2048) return c;
+    for (int c = 2; c <= MXC; c *= 2)
+        if (nx % c == 0) return c;
+    for (int c = MXC; c >= 2; c /= 2)
+        if ((nx + c - 1) / c * tiles >= 2048) return c;
+    return 2;
 }
 bool density_fused(const cpmp_ctx *c) { return c->rho_next && collide_variant() == 2; }
 
@@ -969,7 +989,7 @@ hipError_t launch_mu(cpmp_ctx *c) {
             hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
                                (TS *)c->mu, solid);
         } else {
-            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + MXC - 1) / MXC;
+            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + c->dp.xc - 1) / c->dp.xc;
             hipLaunchKernelGGL(k_mu_march<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
                                (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid);
         }
@@ -1003,7 +1023,7 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
                 hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
                                    fin, fout, rho, mu, c->mv, uout);
         } else {
-            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + MXC - 1) / MXC;
+            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + c->dp.xc - 1) / c->dp.xc;
             const dim3 grid((unsigned)(ntz * nty * nch));
             constexpr bool can_rs = !std::is_same<TF, double>::value;   // FP64 keeps the separate pass
             if (can_rs && rs && want_mv)
@@ -1030,7 +1050,7 @@ hipError_t launch_macro(cpmp_ctx *c) {
     if (density_fused(c)) {   // summed by the collide kernel; add the tile and chunk borders
         return dispatch(c->form, [&](auto fm) {
             using TS = typename decltype(fm)::TS;
-            const int ntz = c->p.nz / MTZ, nty = c->p.ny / MTY, nch = c->p.nx / MXC;
+            const int ntz = c->p.nz / MTZ, nty = c->p.ny / MTY, nch = c->p.nx / c->dp.xc;
             hipLaunchKernelGGL(k_density_fix<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
                                (TS *)c->rho, (const TS *)c->rbuf, (const TS *)c->cbuf, ntz, nty, nch);
             return hipGetLastError();

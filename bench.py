@@ -117,7 +117,7 @@ def measured_traffic(kernel_key):
     the committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE passes calibrated with a
     copy kernel of known size); None if the summary does not list the kernel."""
     import re
-    path = os.path.join(ROOT, "profiles", "step_256cubed_rke_kernel_stats.md")
+    path = os.path.join(ROOT, "profiles", "collide_traffic_256cubed.txt")
     try:
         with open(path) as fh:
             for line in fh:
@@ -132,8 +132,8 @@ def measured_traffic(kernel_key):
 def adjoint_cost(cpmp, nsteps=2000, every=100):
     """adjoint time over forward time, 96x96x64 sessile PR droplet, Tr = 0.75,
     FP64, two schedules: checkpoints every `every` steps with every segment
-    recomputed, and the tape of all steps held in device memory (40 B per node
-    and step: 47 GB for 2000 steps) so that nothing is recomputed."""
+    recomputed, and the tape of all steps held in device memory (80 B per node
+    and step: 94 GB for 2000 steps) so that nothing is recomputed."""
     import numpy as np
     nx, ny, nz = 96, 96, 64
     p = cpmp.make_params(nx, ny, nz, eos="PRW", tr=0.75)

@@ -109,7 +109,7 @@ typedef struct {
     int nsteps;                /* steps of the forward run, starting from the current state */
     int checkpoint_every;      /* steps between stored population checkpoints (>=1) */
     const double *target;      /* rho target, N values (reference layout) */
-    /* Device bytes for the tape (rho, mu, u per node and step, 40 B in FP64) of the
+    /* Device bytes for the tape (rho, u, grad mu, grad rho per node and step, 80 B) of the
        trailing steps recorded during the first forward run; those steps are not
        recomputed.  The steps before them are covered by checkpoints every
        checkpoint_every steps and recomputed segment by segment.  0: checkpoints and

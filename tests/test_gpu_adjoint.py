@@ -217,9 +217,9 @@ def test_adjoint_tape_budget_schedules_agree(cpmp):
     dims = (16, 12, 20)
     kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims)
     n = dims[0] * dims[1] * dims[2]
-    per_step = 5 * n * 8
+    per_step = 10 * n * 8          # rho, u, grad mu, grad rho in FP64
     res = []
-    for budget in (0, (13 + 1) * per_step, -1):
+    for budget in (0, 13 * per_step, -1):
         g = cpmp.Lattice(cpmp.make_params(*dims, **kw))
         g.set_wall_mu(mu_lo, mu_hi)
         g.init_from_rho(rho0)

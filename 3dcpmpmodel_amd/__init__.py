@@ -26,9 +26,10 @@ Q = 19
 EXPORTS = [
     "cpmp_set_multiphase", "cpmp_default_params", "cpmp_eos_pressure", "cpmp_eos_mu",
     "cpmp_coexistence", "cpmp_cocurve_lookup", "cpmp_create", "cpmp_destroy", "cpmp_get_params",
-    "cpmp_memory", "cpmp_init_reference", "cpmp_init_spheres", "cpmp_init_from_rho",
+    "cpmp_memory", "cpmp_init_reference", "cpmp_init_splash", "cpmp_init_spheres", "cpmp_init_from_rho",
     "cpmp_set_wall_mu", "cpmp_step", "cpmp_sync", "cpmp_step_count", "cpmp_get_field",
-    "cpmp_set_field", "cpmp_reduce", "cpmp_contact_angle", "cpmp_cap_fit",
+    "cpmp_set_field", "cpmp_reduce", "cpmp_contact_angle", "cpmp_cap_fit", "cpmp_probe",
+    "cpmp_laplace_jump",
     "cpmp_checkpoint_save", "cpmp_checkpoint_load", "cpmp_write_plt", "cpmp_adjoint",
     "cpmp_kernel_times", "cpmp_time_steps", "cpmp_algorithmic_bytes_per_node",
     "cpmp_last_error", "cpmp_version", "cpmp_selftest_collide",
@@ -92,6 +93,7 @@ def _load():
     L.cpmp_get_params.argtypes = [p, C.POINTER(Params)]
     L.cpmp_memory.argtypes = [p, C.POINTER(C.c_int64)]
     L.cpmp_init_reference.argtypes = [p, d, d, d, d, d]
+    L.cpmp_init_splash.argtypes = [p, d, d, d, d, d]
     L.cpmp_init_spheres.argtypes = [p, C.POINTER(InitDesc)]
     L.cpmp_init_from_rho.argtypes = [p, p]
     L.cpmp_set_wall_mu.argtypes = [p, p, p, C.c_int64]
@@ -108,6 +110,8 @@ def _load():
     L.cpmp_checkpoint_load.argtypes = [p, C.c_char_p]
     L.cpmp_write_plt.argtypes = [p, C.c_char_p, C.c_char_p, i, i]
     L.cpmp_adjoint.argtypes = [p, C.POINTER(AdjDesc), p, p, p, p]
+    L.cpmp_probe.argtypes = [p, i, i, i, i, p]
+    L.cpmp_laplace_jump.argtypes = [p, p, p, p]
     L.cpmp_kernel_times.argtypes = [p, i, p, p]
     L.cpmp_time_steps.argtypes = [p, i, dp]
     L.cpmp_algorithmic_bytes_per_node.argtypes = [p]
@@ -224,6 +228,10 @@ class Lattice:
     def init_reference(self, radius, width, rho_g, rho_l, solid_value=0.0):
         check(lib.cpmp_init_reference(self.h, radius, width, rho_g, rho_l, solid_value))
 
+    def init_splash(self, rho_g, rho_l, radius=50.0, width=10.0, film_z=32.0):
+        """SetFlowField DROPLETSPLASH (FluidCuda.cu:219-233)"""
+        check(lib.cpmp_init_splash(self.h, radius, width, film_z, rho_g, rho_l))
+
     def init_spheres(self, spheres, width, rho_g, rho_l, invert=False, noise_amp=0.0, seed=0,
                      solid_rho=0.0):
         d = InitDesc()
@@ -308,6 +316,21 @@ class Lattice:
         check(lib.cpmp_adjoint(self.h, C.byref(d), _ptr(J), _ptr(glo), _ptr(ghi) if want_hi else None,
                                _ptr(gka)))
         return J[0], glo, ghi, gka[0]
+
+    def probe(self, field, x, y, z):
+        """one node's rho or mu (cpmp_probe)"""
+        fid = FIELD[field] if isinstance(field, str) else field
+        v = np.zeros(1)
+        check(lib.cpmp_probe(self.h, fid, int(x), int(y), int(z), _ptr(v)))
+        return float(v[0])
+
+    def laplace_jump(self, inside, outside):
+        """K^2 [p(rho(inside)) - p(rho(outside))] (cpmp_laplace_jump)"""
+        a = np.ascontiguousarray(inside, np.int32)
+        b = np.ascontiguousarray(outside, np.int32)
+        v = np.zeros(1)
+        check(lib.cpmp_laplace_jump(self.h, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p), _ptr(v)))
+        return float(v[0])
 
     def kernel_times(self, nsteps):
         ms = np.zeros(4)

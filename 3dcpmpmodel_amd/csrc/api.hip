@@ -293,6 +293,18 @@ int cpmp_init_reference(cpmp_ctx *c, double radius, double width, double rg, dou
     return CPMP_OK;
 }
 
+int cpmp_init_splash(cpmp_ctx *c, double radius, double width, double film_z, double rg, double rl) {
+    if (!valid_ctx(c)) return set_error(CPMP_EINVAL, "bad context");
+    if (!(width > 0)) return set_error(CPMP_EINVAL, "width must be > 0");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(launch_init_splash(c, radius, width, film_z, rg, rl));
+    HIPCHK(hipMemsetAsync(c->mv, 0, 3 * c->N * sizeof(double), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->step = 0;
+    c->mv_valid = false;
+    return CPMP_OK;
+}
+
 int cpmp_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d) {
     if (!valid_ctx(c) || !d) return set_error(CPMP_EINVAL, "bad context or descriptor");
     if (d->nspheres < 0 || d->nspheres > 8) return set_error(CPMP_EINVAL, "nspheres %d", d->nspheres);

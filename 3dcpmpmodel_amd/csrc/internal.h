@@ -80,6 +80,7 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout = nullptr);
 hipError_t launch_macro(cpmp_ctx *c);
 hipError_t launch_step(cpmp_ctx *c, bool want_mv);
 hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, double rg, double rl, double solid);
+hipError_t launch_init_splash(cpmp_ctx *c, double radius, double width, double film, double rg, double rl);
 hipError_t launch_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d);
 hipError_t launch_feq_from_rho(cpmp_ctx *c);
 hipError_t launch_pop_from_double(cpmp_ctx *c, int q, const double *src, void *dst_pop);
@@ -87,6 +88,10 @@ hipError_t launch_pop_to_double(cpmp_ctx *c, int q, const void *src_pop, double 
 hipError_t launch_scalar_from_double(cpmp_ctx *c, const double *src, void *dst);
 hipError_t launch_scalar_to_double(cpmp_ctx *c, const void *src, double *dst);
 hipError_t launch_velocity(cpmp_ctx *c, double *out3);
+// XZ plane y = tier of rho into out (nx*nz doubles, out[i*nz + k])
+hipError_t launch_plane_y(cpmp_ctx *c, int tier, double *out);
+// per-block (area, max height) partials of the cap fit into part (2 per block)
+hipError_t launch_cap_columns(cpmp_ctx *c, double mid, double *part, unsigned *nblocks);
 hipError_t launch_force(cpmp_ctx *c, double *out3);
 hipError_t launch_reduce(cpmp_ctx *c, double *host_out /* 12 */);
 hipError_t launch_type(cpmp_ctx *c, double *out);

@@ -278,7 +278,7 @@ struct LdsStencil {
 // Arithmetic of mu_node, as k_mu.
 template <class TS>
 __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
-                                                  int ntz, int nty, TS *__restrict__ rho_solid) {
+                                                  int ntz, int nty, TS *__restrict__ rho_solid, int xc) {
     __shared__ TS s_rho[7][MLP];
     int t = blockIdx.x;
     if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
     const int tyi = t % nty;
     const int ch = t / nty;
     const int y0 = tyi * MTY, z0 = tzi * MTZ;
-    const int xa = ch * P.xc, xb = min(xa + P.xc, P.nx);
+    const int xa = ch * xc, xb = min(xa + xc, P.nx);
     const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
     const int y = y0 + ty, z = z0 + tz;
     const bool inr = (y < P.ny) && (z < P.nz);
@@ -771,6 +771,27 @@ __global__ void __launch_bounds__(TPB) k_init_reference(const DevParams P, TF *_
     store_feq(P, f, nd, d);
 }
 
+// DROPLETSPLASH branch of SetFlowField (FluidCuda.cu:219-233): a droplet centred
+// in the box above a liquid film below z = film; solid rho = mu = 0
+template <class TF, class TS>
+__global__ void __launch_bounds__(TPB) k_init_splash(const DevParams P, TF *__restrict__ f, TS *__restrict__ rho,
+                                                     TS *__restrict__ mu, double radius, double width, double film,
+                                                     double rg, double rl) {
+    const NodeIdx nd = node_from_grid(P);
+    if (!nd.ok) return;
+    const long long I = nd.I;
+    const Coord c{nd.x, nd.y, nd.z};
+    double d = 0;
+    mu[I] = (TS)0;
+    if (!is_solid(P, c.z)) {
+        const double ddx = double(c.x) - P.nx / 2, ddy = double(c.y) - P.ny / 2, ddz = double(c.z) - P.nz / 2;
+        const double r = (double)sqrtf((float)(ddx * ddx + ddy * ddy + ddz * ddz));   // FluidCuda.cu:230
+        d = rl + (rl - rg) / 2 * (tanh((radius - r) * 2 / width) + tanh((film - double(c.z)) * 2 / width));
+    }
+    rho[I] = (TS)d;
+    store_feq(P, f, nd, d);
+}
+
 template <class TF, class TS>
 __global__ void __launch_bounds__(TPB) k_init_spheres(const DevParams P, TF *__restrict__ f, TS *__restrict__ rho,
                                                       TS *__restrict__ mu, const cpmp_init_desc D) {
@@ -830,6 +851,52 @@ __global__ void __launch_bounds__(TPB) k_pop_to_double(const DevParams P, int q,
     const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + nb.site : pull_index(nb, Nu, q);
     dst[nd.I] = pop_load<TF, double>(src, slot, q, P.rho0);
 }
+// ---------------------------------------------------------------------------
+// Observables on the device: the XZ plane y = tier (the contact-angle scan reads
+// only that plane), and per-column sums for the 3-D cap fit.
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_plane_y(const DevParams P, const TS *__restrict__ rho, int tier,
+                                                 double *__restrict__ out) {
+    const long long e = (long long)blockIdx.x * TPB + threadIdx.x;
+    if (e >= (long long)P.nx * P.nz) return;
+    const long long i = e / P.nz, k = e - i * P.nz;
+    out[e] = (double)rho[(i * P.ny + tier) * P.nz + k];
+}
+
+// per column (x, y): 1 when the first fluid layer is at or above the interface
+// density, and the highest interface crossing above the wall plane z = lo - 1/2;
+// block partials (sum, max) in part[2 b], part[2 b + 1]
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_cap_columns(const DevParams P, const TS *__restrict__ rho, double mid,
+                                                     double *__restrict__ part) {
+    const long long col = (long long)blockIdx.x * TPB + threadIdx.x;
+    double area = 0, h = 0;
+    if (col < (long long)P.nx * P.ny) {
+        const TS *c = rho + col * P.nz;
+        const double zw = P.lo - 0.5;
+        if ((double)c[P.lo] >= mid) area = 1;
+        for (int k = P.lo; k < P.nz - P.hi - 1; ++k) {
+            const double a = (double)c[k], b = (double)c[k + 1];
+            if (a >= mid && b < mid) {
+                const double z = k + (a - mid) / (a - b);
+                if (z - zw > h) h = z - zw;
+            }
+        }
+    }
+    const double sa = block_sum(area);
+    __shared__ double smax[TPB / 64];
+    double m = h;
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0;
+        for (int w = 0; w < TPB / 64; ++w) mm = fmax(mm, smax[w]);
+        part[2 * blockIdx.x] = sa;
+        part[2 * blockIdx.x + 1] = mm;
+    }
+}
+
 // raw copy of every population slot between the device layout (pop_site) and the
 // reference node order
 template <class TF>
@@ -989,9 +1056,12 @@ hipError_t launch_mu(cpmp_ctx *c) {
             hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
                                (TS *)c->mu, solid);
         } else {
-            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + c->dp.xc - 1) / c->dp.xc;
+            // a quarter of the collide's chunk: at 7 workgroups per CU the 32-plane
+            // chunks of a 256^3 lattice leave a second round 14 % full
+            const int xc = std::max(2, c->dp.xc / 4);
+            const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + xc - 1) / xc;
             hipLaunchKernelGGL(k_mu_march<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
-                               (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid);
+                               (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc);
         }
         return hipGetLastError();
     });
@@ -1084,6 +1154,16 @@ hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, doubl
     });
 }
 
+hipError_t launch_init_splash(cpmp_ctx *c, double radius, double width, double film, double rg, double rl) {
+    return dispatch(c->form, [&](auto fm) {
+        using TF = typename decltype(fm)::TF;
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_init_splash<TF, TS>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
+                           (TF *)c->f[c->cur], (TS *)c->rho, (TS *)c->mu, radius, width, film, rg, rl);
+        return hipGetLastError();
+    });
+}
+
 hipError_t launch_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
@@ -1144,6 +1224,25 @@ hipError_t launch_scalar_to_double(cpmp_ctx *c, const void *src, double *dst) {
         using TS = typename decltype(fm)::TS;
         hipLaunchKernelGGL((k_convert<TS, double>), dim3(blocks_for(c->N)), dim3(TPB), 0, c->stream, (long long)c->N,
                            (const TS *)src, dst);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_plane_y(cpmp_ctx *c, int tier, double *out) {
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL(k_plane_y<TS>, dim3(blocks_for((long long)c->p.nx * c->p.nz)), dim3(TPB), 0, c->stream,
+                           c->dp, (const TS *)c->rho, tier, out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_cap_columns(cpmp_ctx *c, double mid, double *part, unsigned *nblocks) {
+    *nblocks = blocks_for(c->NXY);
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL(k_cap_columns<TS>, dim3(*nblocks), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, mid,
+                           part);
         return hipGetLastError();
     });
 }

@@ -1,7 +1,11 @@
-// observables.cpp -- droplet observables on the host: the reference's
-// contact-angle measurement and a 3-D spherical-cap fit.
+// observables.cpp -- droplet observables: the reference's contact-angle
+// measurement (on the one XZ plane it scans, extracted on the device), a 3-D
+// spherical-cap fit (column reduction on the device), single-node probes and the
+// Laplace pressure jump.
 #include "internal.h"
+#include "lbm_math.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -9,10 +13,11 @@ using namespace cpmp;
 
 namespace {
 
-int fetch_rho(cpmp_ctx *c, std::vector<double> &rho) {
-    rho.resize(c->N);
-    return cpmp_get_field(c, CPMP_FIELD_RHO, rho.data());
-}
+#define OBS_HIP(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return set_error(CPMP_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
 
 }  // namespace
 
@@ -28,12 +33,14 @@ int cpmp_contact_angle(cpmp_ctx *c, int tier, double rho_g, double rho_l, double
     const int DX = c->p.nx, DY = c->p.ny, DZ = c->p.nz;
     if (tier < 0 || tier >= DY) return set_error(CPMP_EINVAL, "tier %d outside [0,%d)", tier, DY);
     if (DX < 8 || DZ < 8) return set_error(CPMP_EINVAL, "domain too small for the contact-angle scan");
-    std::vector<double> rho;
-    int rc = fetch_rho(c, rho);
-    if (rc) return rc;
+    OBS_HIP(hipSetDevice(c->p.device));
+    std::vector<double> plane((size_t)DX * DZ);
+    OBS_HIP(launch_plane_y(c, tier, c->staging));
+    OBS_HIP(hipMemcpyAsync(plane.data(), c->staging, plane.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    OBS_HIP(hipStreamSynchronize(c->stream));
     const double PI = 3.14159265358979323846264338327950288;
     const double Mid = (rho_g + rho_l) / 2;
-    auto T = [&](int i, int k) { return rho[((int64_t)i * DY + tier) * DZ + k]; };
+    auto T = [&](int i, int k) { return plane[(size_t)i * DZ + k]; };
     double Px[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = 2; j <= 5; ++j) {
         const int n = (j - 2) * 2;
@@ -85,22 +92,18 @@ int cpmp_contact_angle(cpmp_ctx *c, int tier, double rho_g, double rho_l, double
 int cpmp_cap_fit(cpmp_ctx *c, double rho_g, double rho_l, double *out) {
     if (!c || !out) return set_error(CPMP_EINVAL, "bad context or output");
     if (c->p.nwall_lo < 1) return set_error(CPMP_EINVAL, "no bottom wall");
-    std::vector<double> rho;
-    int rc = fetch_rho(c, rho);
-    if (rc) return rc;
-    const int DX = c->p.nx, DY = c->p.ny, DZ = c->p.nz, lo = c->p.nwall_lo, hi = c->p.nwall_hi;
-    const double Mid = (rho_g + rho_l) / 2, zw = lo - 0.5;
+    OBS_HIP(hipSetDevice(c->p.device));
+    const double Mid = (rho_g + rho_l) / 2;
+    unsigned nb = 0;
+    OBS_HIP(launch_cap_columns(c, Mid, c->staging, &nb));
+    std::vector<double> part(2 * (size_t)nb);
+    OBS_HIP(hipMemcpyAsync(part.data(), c->staging, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    OBS_HIP(hipStreamSynchronize(c->stream));
     double area = 0, H = 0;
-    for (int i = 0; i < DX; ++i)
-        for (int j = 0; j < DY; ++j) {
-            const double *col = &rho[((int64_t)i * DY + j) * DZ];
-            if (col[lo] >= Mid) area += 1;
-            for (int k = lo; k < DZ - hi - 1; ++k)
-                if (col[k] >= Mid && col[k + 1] < Mid) {
-                    const double z = k + (col[k] - Mid) / (col[k] - col[k + 1]);
-                    if (z - zw > H) H = z - zw;
-                }
-        }
+    for (unsigned b = 0; b < nb; ++b) {
+        area += part[2 * b];
+        H = std::max(H, part[2 * b + 1]);
+    }
     const double a = std::sqrt(area / 3.14159265358979323846);
     double R = NAN, ang = NAN;
     if (H > 0) {
@@ -108,6 +111,40 @@ int cpmp_cap_fit(cpmp_ctx *c, double rho_g, double rho_l, double *out) {
         ang = std::atan2(a, R - H) * 180.0 / 3.14159265358979323846;
     }
     out[0] = ang; out[1] = H; out[2] = a; out[3] = R;
+    return CPMP_OK;
+}
+
+// One node's density or chemical potential (FP64), without copying the field.
+int cpmp_probe(cpmp_ctx *c, int field, int x, int y, int z, double *value) {
+    if (!c || !c->stream || !value) return set_error(CPMP_EINVAL, "bad context or output");
+    if (field != CPMP_FIELD_RHO && field != CPMP_FIELD_MU) return set_error(CPMP_EINVAL, "field %d: rho or mu only", field);
+    if (x < 0 || x >= c->p.nx || y < 0 || y >= c->p.ny || z < 0 || z >= c->p.nz)
+        return set_error(CPMP_EINVAL, "node (%d,%d,%d) outside the lattice", x, y, z);
+    OBS_HIP(hipSetDevice(c->p.device));
+    const size_t I = ((size_t)x * c->p.ny + y) * c->p.nz + z;
+    const char *src = (const char *)(field == CPMP_FIELD_RHO ? c->rho : c->mu) + I * c->scal_bytes;
+    if (c->scal_bytes == sizeof(double)) {
+        OBS_HIP(hipMemcpyAsync(value, src, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        OBS_HIP(hipStreamSynchronize(c->stream));
+    } else {
+        float v = 0;
+        OBS_HIP(hipMemcpyAsync(&v, src, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        OBS_HIP(hipStreamSynchronize(c->stream));
+        *value = v;
+    }
+    return CPMP_OK;
+}
+
+// Laplace pressure jump K^2 [p(rho_in) - p(rho_out)] between two nodes, with the
+// EOS pressure of EosPressure (FluidCuda.cu:1561-1573, all EOS of the build).
+int cpmp_laplace_jump(cpmp_ctx *c, const int *in_xyz, const int *out_xyz, double *dp) {
+    if (!c || !in_xyz || !out_xyz || !dp) return set_error(CPMP_EINVAL, "bad arguments");
+    double ri = 0, ro = 0;
+    int rc = cpmp_probe(c, CPMP_FIELD_RHO, in_xyz[0], in_xyz[1], in_xyz[2], &ri);
+    if (!rc) rc = cpmp_probe(c, CPMP_FIELD_RHO, out_xyz[0], out_xyz[1], out_xyz[2], &ro);
+    if (rc) return rc;
+    const cpmp_params &p = c->p;
+    *dp = p.K * p.K * (pressure_eos(p.eos, p.A, p.B, p.T, ri) - pressure_eos(p.eos, p.A, p.B, p.T, ro));
     return CPMP_OK;
 }
 

@@ -149,6 +149,11 @@ int cpmp_memory(cpmp_ctx *ctx, int64_t *bytes);
    width `width`, r evaluated with single-precision sqrt as the reference does. */
 int cpmp_init_reference(cpmp_ctx *ctx, double radius, double width, double rho_g, double rho_l,
                         double solid_value);
+/* DROPLETSPLASH branch (FluidCuda.cu:219-233): rho = rho_l + (rho_l-rho_g)/2 *
+   (tanh((radius-r)*2/width) + tanh((film_z-k)*2/width)), r from the box centre in
+   single precision; the reference fixes radius 50, width 10, film_z 32.  Solid rho and
+   mu are 0. */
+int cpmp_init_splash(cpmp_ctx *ctx, double radius, double width, double film_z, double rho_g, double rho_l);
 int cpmp_init_spheres(cpmp_ctx *ctx, const cpmp_init_desc *d);
 /* rho from the host; populations = Feq(rho, 0). */
 int cpmp_init_from_rho(cpmp_ctx *ctx, const double *rho);
@@ -176,6 +181,12 @@ int cpmp_contact_angle(cpmp_ctx *ctx, int tier, double rho_g, double rho_l, doub
 /* 3-D spherical-cap fit of the rho=(rho_g+rho_l)/2 surface above the bottom wall.
    out[4]: angle (degrees), cap height, base radius, sphere radius. */
 int cpmp_cap_fit(cpmp_ctx *ctx, double rho_g, double rho_l, double *out);
+/* Density (CPMP_FIELD_RHO) or chemical potential (CPMP_FIELD_MU) of one node, FP64,
+   without copying the field (the reference reads managed Dens / Pote elements). */
+int cpmp_probe(cpmp_ctx *ctx, int field, int x, int y, int z, double *value);
+/* Laplace pressure jump K^2 [p(rho(in)) - p(rho(out))] between two nodes, with the
+   EOS pressure of EosPressure (FluidCuda.cu:1561-1573) for the context's EOS. */
+int cpmp_laplace_jump(cpmp_ctx *ctx, const int *in_xyz, const int *out_xyz, double *dp);
 
 /* ---- checkpoint / resume (no reference counterpart) ---- */
 int cpmp_checkpoint_save(cpmp_ctx *ctx, const char *path);

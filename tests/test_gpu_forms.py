@@ -106,3 +106,38 @@ def test_write_plt_layout(cpmp, tmp_path):
         rho.tofile(rho_path)
         subprocess.check_call([ref_bin, rho_path, str(nx), str(ny), str(nz), "1", str(tmp_path), "FlowField_5"])
         assert open(str(tmp_path / "FlowField_5.plt"), "rb").read() == raw
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_device_observables_match_host_restatement(cpmp, form):
+    """cpmp_probe, cpmp_cap_fit (column reduction on the device) and
+    cpmp_laplace_jump against a NumPy restatement on the same density field"""
+    case = gc.CASES["ref_prw_walls"]
+    _, g = gc.build(case, cpmp, form)
+    g.step(60)
+    nx, ny, nz = case["dims"]
+    rho = g.get("rho").reshape(nx, ny, nz)
+    mu = g.get("mu").reshape(nx, ny, nz)
+    for x, y, z in [(0, 0, 0), (5, 7, 11), (nx - 1, ny - 1, nz - 1), (nx // 2, ny // 2, 3)]:
+        assert g.probe("rho", x, y, z) == rho[x, y, z]
+        assert g.probe("mu", x, y, z) == mu[x, y, z]
+    dg, dl = gc.densities("PRW", 0.7)
+    mid, lo, hi = (dg + dl) / 2, 3, 3
+    area = float(np.sum(rho[:, :, lo] >= mid))
+    a, b = rho[:, :, lo:nz - hi - 1], rho[:, :, lo + 1:nz - hi]
+    cross = (a >= mid) & (b < mid)
+    k = np.arange(lo, nz - hi - 1)[None, None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        zc = np.where(cross, k + (a - mid) / (a - b), -np.inf)
+    H = max(0.0, float(np.max(zc - (lo - 0.5))))
+    cap = g.cap_fit(dg, dl)
+    assert cap["height"] == pytest.approx(H, rel=1e-14)
+    assert cap["base_radius"] == pytest.approx(np.sqrt(area / np.pi), rel=1e-14)
+    p = g.p
+    dp = g.laplace_jump((nx // 2, ny // 2, 8), (0, 0, nz // 2))
+    want = p.K ** 2 * (cpmp.eos_pressure("PRW", p.A, p.B, p.T, rho[nx // 2, ny // 2, 8])
+                       - cpmp.eos_pressure("PRW", p.A, p.B, p.T, rho[0, 0, nz // 2]))
+    assert dp == pytest.approx(want, rel=1e-13, abs=1e-15)
+    with pytest.raises(cpmp.CPMPError):
+        g.probe("rho", nx, 0, 0)
+    g.close()

@@ -43,6 +43,13 @@ namespace {
 
 bool valid_ctx(cpmp_ctx *c) { return c != nullptr && c->stream != nullptr; }
 
+// FP16 form: after an initial state is written, its density is also the
+// reference density the populations were shifted by
+hipError_t sync_reference_density(cpmp_ctx *c) {
+    if (c->form != CPMP_FORM_FP16) return hipSuccess;
+    return hipMemcpyAsync(c->rho_next, c->rho, c->N * c->scal_bytes, hipMemcpyDeviceToDevice, c->stream);
+}
+
 int pop_size(int form) { return form == CPMP_FORM_FP64 ? 8 : (form == CPMP_FORM_FP32 ? 4 : 2); }
 int scal_size(int form) { return form == CPMP_FORM_FP64 ? 8 : 4; }
 
@@ -237,6 +244,12 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
         c->dp.nty = p->ny / kTileY;
         c->dp.ntz = p->nz / kTileZ;
     }
+    // the FP16 form double-buffers the density on every lattice: the previous
+    // density is the reference its populations are shifted by
+    if (c->form == CPMP_FORM_FP16 && !c->rho_next) {
+        if (hipMalloc(&c->rho_next, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "previous rho");
+        c->fuse_bytes = (size_t)c->N * c->scal_bytes;
+    }
     if (hipMalloc((void **)&c->mv, 3 * c->N * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "MV");
     if (hipMalloc((void **)&c->mu_w, 2 * c->NXY * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "wall mu");
     if (hipMalloc((void **)&c->red, reduce_scratch_doubles() * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "reduce");
@@ -286,6 +299,7 @@ int cpmp_init_reference(cpmp_ctx *c, double radius, double width, double rg, dou
     if (!(width > 0)) return set_error(CPMP_EINVAL, "width must be > 0");
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(launch_init_reference(c, radius, width, rg, rl, solid));
+    HIPCHK(sync_reference_density(c));
     HIPCHK(hipMemsetAsync(c->mv, 0, 3 * c->N * sizeof(double), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->step = 0;
@@ -298,6 +312,7 @@ int cpmp_init_splash(cpmp_ctx *c, double radius, double width, double film_z, do
     if (!(width > 0)) return set_error(CPMP_EINVAL, "width must be > 0");
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(launch_init_splash(c, radius, width, film_z, rg, rl));
+    HIPCHK(sync_reference_density(c));
     HIPCHK(hipMemsetAsync(c->mv, 0, 3 * c->N * sizeof(double), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->step = 0;
@@ -311,6 +326,7 @@ int cpmp_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d) {
     if (!(d->width > 0)) return set_error(CPMP_EINVAL, "width must be > 0");
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(launch_init_spheres(c, d));
+    HIPCHK(sync_reference_density(c));
     HIPCHK(hipMemsetAsync(c->mv, 0, 3 * c->N * sizeof(double), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->step = 0;
@@ -323,6 +339,7 @@ int cpmp_init_from_rho(cpmp_ctx *c, const double *rho) {
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(hipMemcpyAsync(c->staging, rho, c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(launch_scalar_from_double(c, c->staging, c->rho));
+    HIPCHK(sync_reference_density(c));
     HIPCHK(launch_feq_from_rho(c));
     HIPCHK(hipMemsetAsync(c->mv, 0, 3 * c->N * sizeof(double), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -468,14 +485,16 @@ int cpmp_reduce(cpmp_ctx *c, cpmp_stats *out) {
 
 // ---------------------------------------------------------------------------
 // Checkpoint: magic, version, params, step, cur populations in storage form (in
-// the reference node order), rho.
+// the reference node order), rho, wall fields, and in the FP16 form the reference
+// density its populations are shifted by.
 static const char kMagic[8] = {'C', 'P', 'M', 'P', 'C', 'K', 'P', '1'};
 
 int cpmp_checkpoint_save(cpmp_ctx *c, const char *path) {
     if (!valid_ctx(c) || !path) return set_error(CPMP_EINVAL, "bad context or path");
     HIPCHK(hipSetDevice(c->p.device));
     const size_t popset = (size_t)c->N * 19 * c->pop_bytes, sc = (size_t)c->N * c->scal_bytes;
-    std::vector<char> buf(popset + sc + 2 * c->NXY * sizeof(double));
+    const size_t refb = c->form == CPMP_FORM_FP16 ? sc : 0;   // FP16: the reference density
+    std::vector<char> buf(popset + sc + 2 * c->NXY * sizeof(double) + refb);
     // populations in the reference node order whatever the device layout
     const void *pops = c->f[c->cur];
     if (c->dp.tiled) {
@@ -485,6 +504,9 @@ int cpmp_checkpoint_save(cpmp_ctx *c, const char *path) {
     HIPCHK(hipMemcpyAsync(buf.data(), pops, popset, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(buf.data() + popset, c->rho, sc, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(buf.data() + popset + sc, c->mu_w, 2 * c->NXY * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (refb)
+        HIPCHK(hipMemcpyAsync(buf.data() + popset + sc + 2 * c->NXY * sizeof(double), c->rho_next, refb,
+                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     FILE *fp = std::fopen(path, "wb");
     if (!fp) return set_error(CPMP_EIO, "cannot write %s", path);
@@ -505,7 +527,8 @@ int cpmp_checkpoint_load(cpmp_ctx *c, const char *path) {
     int64_t step;
     uint64_t nbytes;
     const size_t popset = (size_t)c->N * 19 * c->pop_bytes, sc = (size_t)c->N * c->scal_bytes;
-    const size_t want = popset + sc + 2 * c->NXY * sizeof(double);
+    const size_t refb = c->form == CPMP_FORM_FP16 ? sc : 0;
+    const size_t want = popset + sc + 2 * c->NXY * sizeof(double) + refb;
     bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, kMagic, 8) == 0;
     if (!ok) { std::fclose(fp); return set_error(CPMP_EIO, "%s is not a checkpoint (bad magic)", path); }
     ok = std::fread(&p, sizeof p, 1, fp) == 1 && std::fread(&step, sizeof step, 1, fp) == 1 &&
@@ -528,6 +551,9 @@ int cpmp_checkpoint_load(cpmp_ctx *c, const char *path) {
     }
     HIPCHK(hipMemcpyAsync(c->rho, buf.data() + popset, sc, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->mu_w, buf.data() + popset + sc, 2 * c->NXY * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (refb)
+        HIPCHK(hipMemcpyAsync(c->rho_next, buf.data() + popset + sc + 2 * c->NXY * sizeof(double), refb,
+                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->step = step;
     c->mv_valid = false;

@@ -5,7 +5,8 @@
 // populations SoA f[q*N + I].  Node types are implicit: solid iff z lies in the
 // wall layers (the geometry of SetFlowField, FluidCuda.cu:204), so no type array
 // is read.  Storage forms: FP64 (double f, rho, mu), FP32 (float), FP16
-// (f stored as f - w_q*rho0 in _Float16; rho, mu float).
+// (f stored as f - w_q*rho_ref in _Float16, rho_ref the density that the node
+// pulling the value had in the step that wrote it; rho, mu float).
 #include "internal.h"
 #include "lbm_math.h"
 #include "device_common.h"
@@ -42,15 +43,26 @@ template <class TC> __device__ __forceinline__ const TC *rates(const DevParams &
 template <> __device__ __forceinline__ const double *rates<double>(const DevParams &P) { return P.ts; }
 template <> __device__ __forceinline__ const float *rates<float>(const DevParams &P) { return P.tsf; }
 
-template <class TF, class TC>
-__device__ __forceinline__ TC pop_load(const TF *p, long long i, int q, double rho0) {
-    if constexpr (kShifted<TF>) return (TC)p[i] + (TC)(wq(q) * rho0);
+// FP16 storage holds f_q - w_q rho_ref with rho_ref the density of the node that
+// pulls the value, in the step that wrote it (the density the collide kernel read
+// there); the stored numbers are then the small deviations from rest, not w_q rho.
+// Solid nodes' own slots are stored unshifted (rho_ref = 0).  The other forms
+// store f_q and ignore rho_ref.
+template <class TF, class TC, class R>
+__device__ __forceinline__ TC pop_load(const TF *p, long long i, int q, R rho_ref) {
+    if constexpr (kShifted<TF>) return (TC)p[i] + (TC)wq(q) * (TC)rho_ref;
     else return (TC)p[i];
 }
-template <class TF, class TC>
-__device__ __forceinline__ TF pop_val(TC v, int q, double rho0) {
-    if constexpr (kShifted<TF>) return (TF)(v - (TC)(wq(q) * rho0));
+template <class TF, class TC, class R>
+__device__ __forceinline__ TF pop_val(TC v, int q, R rho_ref) {
+    if constexpr (kShifted<TF>) return (TF)(v - (TC)wq(q) * (TC)rho_ref);
     else return (TF)v;
+}
+// rho_ref of a pulling node from the previous-density buffer (FP16 form only)
+template <class TF, class TS>
+__device__ __forceinline__ TS ref_at(const TS *rref, long long I) {
+    if constexpr (kShifted<TF>) return rref[I];
+    else return TS(0);
 }
 
 
@@ -191,7 +203,8 @@ __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restr
 template <class TF, class TS, class TC, bool MV>
 __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__restrict__ fin, TF *__restrict__ fout,
                                                  const TS *__restrict__ rho, const TS *__restrict__ mu,
-                                                 double *__restrict__ mv, double *__restrict__ uout) {
+                                                 double *__restrict__ mv, double *__restrict__ uout,
+                                                 const TS *__restrict__ rref) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
     const long long I = nd.I;
@@ -211,8 +224,9 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
     const unsigned Iu = nb.site;
 
     TC f[Q];
+    const TS ref = ref_at<TF>(rref, I);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, pull_index(nb, Nu, q), q, P.rho0);
+    for (int q = 0; q < Q; ++q) f[q] = pop_load<TF, TC>(fin, pull_index(nb, Nu, q), q, ref);
 
     const TC r = (TC)rho[I];
     TC gmx, gmy, gmz, grx, gry, grz;
@@ -239,9 +253,17 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
         mv[N + I] = (double)(u[1] + Fy * h);
         mv[2 * N + I] = (double)(u[2] + Fz * h);
     }
-    // post-collision values stay at this node; the next step pulls them
+    // post-collision values stay at this node; the next step pulls them (FP16:
+    // shifted by the density the puller has now, or this node's when bounced back)
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fout[(unsigned)q * Nu + Iu] = pop_val<TF, TC>(out[q], q, P.rho0);
+    for (int q = 0; q < Q; ++q) {
+        TS tref = TS(0);
+        if constexpr (kShifted<TF>) {
+            const int tz = zs[3 + cz(q)];
+            tref = is_solid(P, tz) ? rho[I] : rho[((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz];
+        }
+        fout[(unsigned)q * Nu + Iu] = pop_val<TF, TC>(out[q], q, tref);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -365,7 +387,7 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
                                                        double *__restrict__ uout, int ntz, int nty,
                                                        TS *__restrict__ rho_next, TS *__restrict__ rbuf,
-                                                       TS *__restrict__ cbuf, int remap) {
+                                                       TS *__restrict__ cbuf, int remap, const TS *__restrict__ rref) {
     __shared__ TS s_rho[7][MLP];
     __shared__ TS s_mu[7][MLP];
     __shared__ TC s_mail[RS ? Q - 3 : 1][RS ? TPB : 1];   // in-plane populations 3..18 of plane x
@@ -420,8 +442,9 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
         nb.site = (unsigned)xg * plane + own_site;
         nb.ox[0] = (wrap_off(xg, -1, P.nx) - xg) * (int)plane;
         nb.ox[2] = (wrap_off(xg, 1, P.nx) - xg) * (int)plane;
+        const TS ref = ref_at<TF>(rref, ((long long)xg * P.ny + ym) * P.nz + zm);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) dst[q] = pop_load<TF, TC>(fin, pull_index(nb, (unsigned)N, q), q, P.rho0);
+        for (int q = 0; q < Q; ++q) dst[q] = pop_load<TF, TC>(fin, pull_index(nb, (unsigned)N, q), q, ref);
     };
     // software pipeline: the populations of plane x+1 and the halo of plane x+4
     // are in flight while plane x computes
@@ -530,7 +553,13 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
             // aligned stores: the next step pulls these post-collision values
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const TF sv = pop_val<TF, TC>(out[q], q, P.rho0);
+                // FP16: shifted by the density its puller has in this step (this
+                // node's when bounced back), read from the LDS ring
+                const bool bounced = (cz(q) < 0 && bb_m) || (cz(q) > 0 && bb_p);
+                TC tref = TC(0);
+                if constexpr (kShifted<TF>)
+                    tref = bounced ? r : (TC)s_rho[slot[3 + cx(q)]][base + cy(q) * MLZ + cz(q)];
+                const TF sv = pop_val<TF, TC>(out[q], q, tref);
                 // streaming store: the populations are read again only next step,
                 // after far more data than the caches hold has passed through
                 // (2-byte values keep ordinary stores, which merge in the cache)
@@ -540,8 +569,7 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
                 if (RS) {
                     // the value the next step pulls: as stored (and decoded)
                     TC w;
-                    if constexpr (kShifted<TF>) w = (TC)sv + (TC)(wq(q) * P.rho0);
-                    else w = (TC)sv;
+                    w = pop_load<TF, TC>(&sv, 0, q, tref);
                     const int qx = cx(q), qy = cy(q), qz = cz(q);
                     if (qy == 0 && qz == 0) {
                         if (qx == 0) acc_cur += w;
@@ -669,22 +697,26 @@ __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__re
 
 // ---------------------------------------------------------------------------
 // Density (MacroCalculate, FluidCuda.cu:351-372): rho = sum_q f_q on fluid nodes.
+// rho_in: the density of the step that wrote f (FP16 decoding); rho_out may be rho_in
 template <class TF, class TS, class TC>
-__global__ void __launch_bounds__(TPB) k_macro(const DevParams P, const TF *__restrict__ f, TS *__restrict__ rho) {
+__global__ void __launch_bounds__(TPB) k_macro_into(const DevParams P, const TF *__restrict__ f, const TS *rho_in,
+                                                    TS *rho_out) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
     const long long I = nd.I;
     if (is_solid(P, nd.z)) return;
     const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
+    const TS ref = ref_at<TF>(rho_in, I);
     TC s = TC(0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s += pop_load<TF, TC>(f, pull_index(nb, (unsigned)P.N, q), q, P.rho0);
-    rho[I] = (TS)s;
+    for (int q = 0; q < Q; ++q) s += pop_load<TF, TC>(f, pull_index(nb, (unsigned)P.N, q), q, ref);
+    rho_out[I] = (TS)s;
 }
 
 // velocity u = j/rho, rho = sum f (fluid nodes; 0 on solid)
-template <class TF, class TC>
-__global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *__restrict__ fp, double *__restrict__ out) {
+template <class TF, class TS, class TC>
+__global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *__restrict__ fp, double *__restrict__ out,
+                                                  const TS *__restrict__ rref) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
     const long long I = nd.I;
@@ -693,7 +725,7 @@ __global__ void __launch_bounds__(TPB) k_velocity(const DevParams P, const TF *_
     TC f[Q], s = TC(0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        f[q] = pop_load<TF, TC>(fp, pull_index(nb, (unsigned)P.N, q), q, P.rho0);
+        f[q] = pop_load<TF, TC>(fp, pull_index(nb, (unsigned)P.N, q), q, ref_at<TF>(rref, I));
         s += f[q];
     }
     // the same arithmetic as the velocity of collide(): moment rows 3, 5, 7 times
@@ -740,13 +772,15 @@ template <class TF>
 __device__ __forceinline__ void store_feq(const DevParams &P, TF *f, const NodeIdx &nd, double d) {
     const unsigned Nu = (unsigned)P.N;
     const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
-    if (is_solid(P, nd.z)) {
+    if (is_solid(P, nd.z)) {   // own slots, unshifted
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[(unsigned)q * Nu + nb.site] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+        for (int q = 0; q < Q; ++q) f[(unsigned)q * Nu + nb.site] = pop_val<TF, double>(d * wq(q), q, 0.0);
         return;
     }
+    // FP16: shifted by this node's density (the previous-density buffer is set to
+    // the initial density by the caller), so the stored deviations are 0
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[pull_index(nb, Nu, q)] = pop_val<TF, double>(d * wq(q), q, P.rho0);
+    for (int q = 0; q < Q; ++q) f[pull_index(nb, Nu, q)] = pop_val<TF, double>(d * wq(q), q, d);
 }
 
 template <class TF, class TS>
@@ -831,25 +865,27 @@ __global__ void __launch_bounds__(TPB) k_feq_from_rho(const DevParams P, TF *__r
 // post-stream populations (the reference's Dist after the swap, FluidCuda.cu:1290);
 // storage is the pull convention, so fluid nodes un-stream / gather through
 // pull_index and solid nodes keep their own slots.
-template <class TF>
+template <class TF, class TS>
 __global__ void __launch_bounds__(TPB) k_pop_from_double(const DevParams P, int q, const double *__restrict__ src,
-                                                         TF *__restrict__ dst) {
+                                                         TF *__restrict__ dst, const TS *__restrict__ rref) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
     const unsigned Nu = (unsigned)P.N;
     const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
-    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + nb.site : pull_index(nb, Nu, q);
-    dst[slot] = pop_val<TF, double>(src[nd.I], q, P.rho0);
+    const bool solid = is_solid(P, nd.z);
+    const unsigned slot = solid ? (unsigned)q * Nu + nb.site : pull_index(nb, Nu, q);
+    dst[slot] = pop_val<TF, double>(src[nd.I], q, solid ? 0.0 : (double)ref_at<TF>(rref, nd.I));
 }
-template <class TF>
+template <class TF, class TS>
 __global__ void __launch_bounds__(TPB) k_pop_to_double(const DevParams P, int q, const TF *__restrict__ src,
-                                                       double *__restrict__ dst) {
+                                                       double *__restrict__ dst, const TS *__restrict__ rref) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
     const unsigned Nu = (unsigned)P.N;
     const Nbr nb = make_nbr(P, nd.x, nd.y, nd.z);
-    const unsigned slot = is_solid(P, nd.z) ? (unsigned)q * Nu + nb.site : pull_index(nb, Nu, q);
-    dst[nd.I] = pop_load<TF, double>(src, slot, q, P.rho0);
+    const bool solid = is_solid(P, nd.z);
+    const unsigned slot = solid ? (unsigned)q * Nu + nb.site : pull_index(nb, Nu, q);
+    dst[nd.I] = pop_load<TF, double>(src, slot, q, solid ? 0.0 : (double)ref_at<TF>(rref, nd.I));
 }
 // ---------------------------------------------------------------------------
 // Observables on the device: the XZ plane y = tier (the contact-angle scan reads
@@ -1046,19 +1082,22 @@ int march_chunk(int nx, int ny, int nz) {
         if ((nx + c - 1) / c * tiles >= 2048) return c;
     return 2;
 }
-bool density_fused(const cpmp_ctx *c) { return c->rho_next && collide_variant() == 2; }
+bool density_fused(const cpmp_ctx *c) { return c->rbuf && collide_variant() == 2; }
+
+// previous-density buffer the FP16 form shifts its populations by (null otherwise)
+template <class TS> static const TS *ref_buffer(const cpmp_ctx *c) {
+    return c->form == CPMP_FORM_FP16 ? (const TS *)c->rho_next : nullptr;
+}
 
 hipError_t launch_mu(cpmp_ctx *c) {
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
-        TS *solid = density_fused(c) ? (TS *)c->rho_next : nullptr;
+        TS *solid = (TS *)c->rho_next;   // double-buffered density: solid values follow the step
         if (mu_variant() == 1) {
             hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
                                (TS *)c->mu, solid);
         } else {
-            // a quarter of the collide's chunk: at 7 workgroups per CU the 32-plane
-            // chunks of a 256^3 lattice leave a second round 14 % full
-            const int xc = std::max(2, c->dp.xc / 4);
+            const int xc = c->dp.xc;   // shorter chunks measured slower (0.077 vs 0.071 ms at 256^3)
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + xc - 1) / xc;
             hipLaunchKernelGGL(k_mu_march<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
                                (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc);
@@ -1088,26 +1127,26 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
         if (collide_variant() == 1) {
             if (want_mv)
                 hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout);
+                                   fin, fout, rho, mu, c->mv, uout, ref_buffer<TS>(c));
             else
                 hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout);
+                                   fin, fout, rho, mu, c->mv, uout, ref_buffer<TS>(c));
         } else {
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + c->dp.xc - 1) / c->dp.xc;
             const dim3 grid((unsigned)(ntz * nty * nch));
             constexpr bool can_rs = !std::is_same<TF, double>::value;   // FP64 keeps the separate pass
             if (can_rs && rs && want_mv)
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
             else if (can_rs && rs)
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
             else if (want_mv)
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, false>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
             else
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, false>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap);
+                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
         }
         return hipGetLastError();
     });
@@ -1126,14 +1165,19 @@ hipError_t launch_macro(cpmp_ctx *c) {
             return hipGetLastError();
         });
     }
-    return dispatch(c->form, [&](auto fm) {
+    // separate density pass: in place, or (FP16, double-buffered) into the other
+    // buffer, which then becomes current while this one keeps the reference density
+    const hipError_t e = dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
         using TS = typename decltype(fm)::TS;
         using TC = typename decltype(fm)::TC;
-        hipLaunchKernelGGL((k_macro<TF, TS, TC>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                           (const TF *)c->f[c->cur], (TS *)c->rho);
+        TS *out = c->rho_next ? (TS *)c->rho_next : (TS *)c->rho;
+        hipLaunchKernelGGL((k_macro_into<TF, TS, TC>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
+                           (const TF *)c->f[c->cur], (const TS *)c->rho, out);
         return hipGetLastError();
     });
+    if (c->rho_next) std::swap(c->rho, c->rho_next);
+    return e;
 }
 
 hipError_t launch_step(cpmp_ctx *c, bool want_mv) {
@@ -1187,7 +1231,9 @@ hipError_t launch_feq_from_rho(cpmp_ctx *c) {
 hipError_t launch_pop_from_double(cpmp_ctx *c, int q, const double *src, void *dst) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
-        hipLaunchKernelGGL(k_pop_from_double<TF>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, q, src, (TF *)dst);
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_pop_from_double<TF, TS>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, q, src,
+                           (TF *)dst, ref_buffer<TS>(c));
         return hipGetLastError();
     });
 }
@@ -1195,8 +1241,9 @@ hipError_t launch_pop_from_double(cpmp_ctx *c, int q, const double *src, void *d
 hipError_t launch_pop_to_double(cpmp_ctx *c, int q, const void *src, double *dst) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
-        hipLaunchKernelGGL(k_pop_to_double<TF>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, q,
-                           (const TF *)src, dst);
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL((k_pop_to_double<TF, TS>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, q,
+                           (const TF *)src, dst, ref_buffer<TS>(c));
         return hipGetLastError();
     });
 }
@@ -1250,9 +1297,10 @@ hipError_t launch_cap_columns(cpmp_ctx *c, double mid, double *part, unsigned *n
 hipError_t launch_velocity(cpmp_ctx *c, double *out3) {
     return dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
+        using TS = typename decltype(fm)::TS;
         using TC = typename decltype(fm)::TC;
-        hipLaunchKernelGGL((k_velocity<TF, TC>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                           (const TF *)c->f[c->cur], out3);
+        hipLaunchKernelGGL((k_velocity<TF, TS, TC>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
+                           (const TF *)c->f[c->cur], out3, ref_buffer<TS>(c));
         return hipGetLastError();
     });
 }

@@ -48,7 +48,9 @@ extern "C" {
 /* ---- storage / arithmetic forms ---- */
 #define CPMP_FORM_FP64 0  /* FP64 populations, FP64 arithmetic */
 #define CPMP_FORM_FP32 1  /* FP32 populations, FP32 arithmetic, FP64 EOS mu and mass sums */
-#define CPMP_FORM_FP16 2  /* FP16 populations stored as f - w_i*rho0, FP32 arithmetic */
+#define CPMP_FORM_FP16 2  /* FP16 populations stored as f_i - w_i*rho_ref, FP32 arithmetic;
+                             rho_ref = density the pulling node had in the step that wrote
+                             the value (the reference shift w_i*rho0, made per node) */
 
 /* ---- field ids for get/set ---- */
 #define CPMP_FIELD_TYPE  0  /* 1 fluid, 2 solid (Common.h:37-38); get only */
@@ -75,7 +77,7 @@ typedef struct {
     double A, B, T;            /* EOS constants (SetMultiphase; T = Tr*Tc) */
     double K, Ka, K1, K2;      /* EOS scale, kappa, surface-tension tuning (FluidCuda.cu:124-127) */
     double Ts[19];             /* MRT relaxation rates (FluidCuda.cu:154-157) */
-    double rho0;               /* FP16 form: populations are stored as f_i - w_i*rho0 */
+    double rho0;               /* not used: the FP16 form takes its shift density per node */
 } cpmp_params;
 
 /* Generic initial condition: up to 8 spheres with tanh interfaces, optional

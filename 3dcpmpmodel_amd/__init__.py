@@ -24,7 +24,7 @@ Q = 19
 
 # every symbol include/cpmp.h declares
 EXPORTS = [
-    "cpmp_set_multiphase", "cpmp_default_params", "cpmp_eos_pressure", "cpmp_eos_mu",
+    "cpmp_set_multiphase", "cpmp_default_params", "cpmp_dim_conversion", "cpmp_eos_pressure", "cpmp_eos_mu",
     "cpmp_coexistence", "cpmp_cocurve_lookup", "cpmp_create", "cpmp_destroy", "cpmp_get_params",
     "cpmp_memory", "cpmp_init_reference", "cpmp_init_splash", "cpmp_init_spheres", "cpmp_init_from_rho",
     "cpmp_set_wall_mu", "cpmp_step", "cpmp_sync", "cpmp_step_count", "cpmp_get_field",
@@ -82,6 +82,7 @@ def _load():
     dp = C.POINTER(C.c_double)
     L.cpmp_set_multiphase.argtypes = [i, d, dp, dp, dp, dp]
     L.cpmp_default_params.argtypes = [C.POINTER(Params), i, i, i, i, d]
+    L.cpmp_dim_conversion.argtypes = [d, d, d, d, d, d, dp]
     L.cpmp_eos_pressure.argtypes = [i, d, d, d, d]
     L.cpmp_eos_pressure.restype = d
     L.cpmp_eos_mu.argtypes = [i, i, d, d, d, d]
@@ -179,6 +180,13 @@ def cocurve_lookup(path, tr):
     g, l = C.c_double(), C.c_double()
     check(lib.cpmp_cocurve_lookup(path.encode(), tr, C.byref(g), C.byref(l)))
     return g.value, l.value
+
+
+def dim_conversion(length, macro_length, density, macro_density, viscosity, macro_viscosity):
+    """DimConversion (FluidCuda.cu:474-486): dict of the lattice-to-physical factors"""
+    out = (C.c_double * 9)()
+    check(lib.cpmp_dim_conversion(length, macro_length, density, macro_density, viscosity, macro_viscosity, out))
+    return dict(zip(["len", "time", "mass", "force", "torque", "velocity", "acceleration", "omega", "beta"], list(out)))
 
 
 def eos_pressure(eos, A, B, T, rho):

@@ -48,6 +48,14 @@ def build(verbose=False):
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed on probe_copy.hip")
+    # the reference program's driver on the C ABI (host C++ only)
+    drv = os.path.join(HERE, "cpmp_sweep")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(HERE, "driver", "cpmp_sweep.cpp"), "-L", HERE, "-lcpmp", "-Wl,-rpath,$ORIGIN", "-o", drv]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed on cpmp_sweep.cpp")
     return OUT
 
 

@@ -120,6 +120,28 @@ int cpmp_set_multiphase(int eos, double Tr, double *A, double *B, double *Tc, do
     return CPMP_OK;
 }
 
+// DimConversion (FluidCuda.cu:474-486): lattice-to-physical unit factors from a
+// length, density and viscosity pair.  out: DimLen, DimTime, DimMass, DimF, DimT,
+// DimV, DimA, DimW, DimB.
+int cpmp_dim_conversion(double length, double macro_length, double density, double macro_density, double viscosity,
+                        double macro_viscosity, double *out) {
+    if (!out || !(length > 0) || !(density > 0) || !(macro_viscosity > 0))
+        return set_error(CPMP_EINVAL, "lengths, densities and the physical viscosity must be > 0");
+    const double len = macro_length / length;
+    const double tim = viscosity / macro_viscosity * len * len;
+    const double mass = macro_density / density * len * len * len;
+    out[0] = len;
+    out[1] = tim;
+    out[2] = mass;
+    out[3] = mass * len / (tim * tim);
+    out[4] = mass * len * len / (tim * tim);
+    out[5] = len / tim;
+    out[6] = len / (tim * tim);
+    out[7] = 1.0 / tim;
+    out[8] = 1.0 / (tim * tim);
+    return CPMP_OK;
+}
+
 int cpmp_default_params(cpmp_params *p, int nx, int ny, int nz, int eos, double Tr) {
     // Initialize(), FluidCuda.cu:119-163
     if (!p) return set_error(CPMP_EINVAL, "params is NULL");

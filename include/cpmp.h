@@ -127,6 +127,11 @@ int cpmp_set_multiphase(int eos, double Tr, double *A, double *B, double *Tc, do
 /* Fill *p with the reference defaults of Initialize() (FluidCuda.cu:119-163) for
    the given domain, EOS and Tr. */
 int cpmp_default_params(cpmp_params *p, int nx, int ny, int nz, int eos, double Tr);
+/* DimConversion (FluidCuda.cu:474-486): unit factors from (lattice, physical) length,
+   density and viscosity.  out[9]: DimLen, DimTime, DimMass, DimF, DimT, DimV, DimA,
+   DimW, DimB. */
+int cpmp_dim_conversion(double length, double macro_length, double density, double macro_density,
+                        double viscosity, double macro_viscosity, double *out);
 /* EOS pressure (FluidCuda.cu:1561-1573, extended to all EOS) and chemical potential
    (FluidCuda.cu:453 for PR) of the bulk, without the K^2 factor. */
 double cpmp_eos_pressure(int eos, double A, double B, double T, double rho);

@@ -96,3 +96,21 @@ def test_invalid_params_rejected(cpmp):
     with pytest.raises(cpmp.CPMPError) as e:
         cpmp.Lattice(p)
     assert e.value.code in (-1, -7)
+
+
+def test_dim_conversion_is_dimconversion(cpmp):
+    """DimConversion (FluidCuda.cu:474-486) with a lattice of 100 nodes for 1 mm,
+    water-like density and viscosity; the factors obey their definitions"""
+    L, ML, D, MD, V, MV = 100.0, 1e-3, 1.0, 1000.0, 0.1, 1e-6
+    d = cpmp.dim_conversion(L, ML, D, MD, V, MV)
+    dl = ML / L
+    dt = V / MV * dl * dl
+    dm = MD / D * dl ** 3
+    want = dict(len=dl, time=dt, mass=dm, force=dm * dl / dt ** 2, torque=dm * dl * dl / dt ** 2,
+                velocity=dl / dt, acceleration=dl / dt ** 2, omega=1 / dt, beta=1 / dt ** 2)
+    for k, v in want.items():
+        assert d[k] == pytest.approx(v, rel=1e-15), k
+    # a lattice viscosity of 0.1 at dx = 10 um in water gives dt = 1e-5 s
+    assert d["time"] == pytest.approx(1e-5, rel=1e-12)
+    with pytest.raises(cpmp.CPMPError):
+        cpmp.dim_conversion(0.0, ML, D, MD, V, MV)

@@ -234,6 +234,68 @@ __global__ void __launch_bounds__(TPB) k_adj_wall_pass(const DevParams P, double
     }
 }
 
+// The transposed fill in one launch for walls of at most R (<= 3) layers, over
+// 16 x 16 tiles of columns (blockIdx.z: 0 bottom wall, 1 top wall).  Layered
+// scheme: the first fluid layer gains A(rb(1) + A(rb(2) + A rb(3))), rb(d) the
+// cotangent of the solid layer at depth d and A the symmetric diagonal average;
+// layer d is staged in LDS on the tile plus a halo of d columns and the sums are
+// formed in the order of the pass kernel above (deepest first), so the results
+// are identical.  Simple scheme: the fluid node gains the sum of A rb(d), deepest
+// first.  Only the fluid layer is written, and only at the tile's own columns.
+template <int R>
+__global__ void __launch_bounds__(kWallTile * kWallTile) k_adj_wall_tile(const DevParams P, double *__restrict__ rb,
+                                                                          const double *__restrict__ mub,
+                                                                          double *__restrict__ glo,
+                                                                          double *__restrict__ ghi) {
+    constexpr int T = kWallTile, E = T + 2 * R, NT = T * T;
+    __shared__ double L[R + 1][E * E];
+    const int top = blockIdx.z;
+    const int depth = top ? P.hi : P.lo;
+    if (depth == 0) return;   // uniform over the workgroup
+    const int kf = top ? P.nz - 1 - P.hi : P.lo, step = top ? -1 : 1;
+    const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+    const int x = x0 + (int)threadIdx.x % T, y = y0 + (int)threadIdx.x / T;
+    const bool own = x < P.nx && y < P.ny;
+    auto at = [&](int xx, int yy, int k) { return ((long long)wrapc(xx, P.nx) * P.ny + wrapc(yy, P.ny)) * P.nz + k; };
+    const long long col = (long long)x * P.ny + y;
+    if (own) {   // wall-field gradient: solid mu = mu_w of the column
+        const double *m = mub + col * P.nz;
+        double g = 0;
+        if (top) for (int k = P.nz - P.hi; k < P.nz; ++k) g += m[k];
+        else for (int k = 0; k < P.lo; ++k) g += m[k];
+        (top ? ghi : glo)[col] += g;
+    }
+    if (!P.wall_scheme) {
+        if (!own) return;
+        double acc = rb[at(x, y, kf)];
+        for (int d = depth; d >= 1; --d) {
+            const int k = kf - d * step;
+            acc += (rb[at(x, y, k)] * 2 + rb[at(x - 1, y - 1, k)] + rb[at(x - 1, y + 1, k)] + rb[at(x + 1, y - 1, k)]
+                    + rb[at(x + 1, y + 1, k)]) / 6;
+        }
+        rb[at(x, y, kf)] = acc;
+        return;
+    }
+    // layer d on the patch [R - d, E - R + d)^2
+    for (int d = 0; d <= depth; ++d)
+        for (int i = threadIdx.x; i < E * E; i += NT) {
+            const int a = i % E, b = i / E;
+            if (a < R - d || a >= E - R + d || b < R - d || b >= E - R + d) continue;
+            L[d][i] = rb[at(x0 - R + a, y0 - R + b, kf - d * step)];
+        }
+    __syncthreads();
+    for (int d = depth - 1; d >= 0; --d) {
+        const double *c = L[d + 1];
+        for (int i = threadIdx.x; i < E * E; i += NT) {
+            const int a = i % E, b = i / E;
+            if (a < R - d || a >= E - R + d || b < R - d || b >= E - R + d) continue;
+            L[d][i] += (c[i] * 2 + c[i - 1 - E] + c[i - 1 + E] + c[i + 1 - E] + c[i + 1 + E]) / 6;
+        }
+        __syncthreads();
+    }
+    if (own) rb[at(x, y, kf)] = L[0][(y - y0 + R) * E + (x - x0 + R)];
+}
+
 // acc[0] += sum(part[0..nb)) in a fixed order
 __global__ void __launch_bounds__(TPB) k_sum_into(const double *__restrict__ part, long long nb, double *__restrict__ acc) {
     double s = 0;
@@ -265,6 +327,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const long long nbg = (long long)ng.x * ng.y * ng.z;   // blocks of the node grid
     const hipStream_t s = c->stream;
     const DevParams &P = c->dp;
+    const dim3 gwt((P.nx + kWallTile - 1) / kWallTile, (P.ny + kWallTile - 1) / kWallTile, 2), bwt(kWallTile * kWallTile);
     const int passes = std::max(P.lo, P.hi);
     const int64_t step0 = c->step;
     double *ckpt = nullptr, *tape = nullptr, *buf = nullptr, *tgt = nullptr;
@@ -328,9 +391,16 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
                 hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
                                    (const double *)rbl, mub, rbn);
                 hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, kpart);
-                for (int pass = passes - 1; pass >= 0; --pass)
-                    hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass,
-                                       pass == passes - 1 ? (const double *)mub : nullptr, gl, gh);
+                if (passes == 1)
+                    hipLaunchKernelGGL(k_adj_wall_tile<1>, gwt, bwt, 0, s, P, rbn, (const double *)mub, gl, gh);
+                else if (passes == 2)
+                    hipLaunchKernelGGL(k_adj_wall_tile<2>, gwt, bwt, 0, s, P, rbn, (const double *)mub, gl, gh);
+                else if (passes == 3)
+                    hipLaunchKernelGGL(k_adj_wall_tile<3>, gwt, bwt, 0, s, P, rbn, (const double *)mub, gl, gh);
+                else
+                    for (int pass = passes - 1; pass >= 0; --pass)
+                        hipLaunchKernelGGL(k_adj_wall_pass, dim3(nbc), dim3(TPB), 0, s, P, rbn, pass,
+                                           pass == passes - 1 ? (const double *)mub : nullptr, gl, gh);
                 const hipError_t e = hipGetLastError();
                 if (e != hipSuccess) return e;
                 std::swap(fbA, fbB);

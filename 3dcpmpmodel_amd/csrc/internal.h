@@ -73,6 +73,7 @@ bool density_fused(const cpmp_ctx *c);
 bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *chunk_elems);
 int march_chunk(int nx, int ny, int nz);
 constexpr int kTileY = 8, kTileZ = 32;   // in-plane tile of the march kernels and of tiled population storage
+constexpr int kWallTile = 16;             // columns per edge of the wall-fill tiles (16 x 16 workgroups)
 // copy one population set between the device layout and the reference node order
 hipError_t launch_pop_relayout(cpmp_ctx *c, const void *src, void *dst, bool to_device_layout);
 // uout (when given): u, grad mu, grad rho of every node, 9 N doubles (adjoint tape)

@@ -99,6 +99,54 @@ __global__ void __launch_bounds__(TPB) k_wall(const DevParams P, TS *__restrict_
     }
 }
 
+// The same fill in one launch for walls of at most R (<= 3) layers.  With the
+// layered scheme layer lo-d is A^d applied to the first fluid layer, A the diagonal
+// average above, so a 16 x 16 tile of columns needs the fluid layer on the tile
+// plus a halo of R columns.  The workgroup stages that patch in LDS and recomputes
+// the layers on shrinking patches with the same arithmetic (including the rounding
+// to the storage type) as the pass kernels, so the results are identical.
+// blockIdx.z selects the wall (0: bottom, 1: top).
+
+template <int R, class TS>
+__global__ void __launch_bounds__(kWallTile * kWallTile) k_wall_tile(const DevParams P, TS *__restrict__ rho,
+                                                                      TS *__restrict__ mu, const double *__restrict__ muw) {
+    constexpr int T = kWallTile, E = T + 2 * R, NT = T * T;
+    __shared__ double buf[2][E * E];
+    const int top = blockIdx.z;
+    const int depth = top ? P.hi : P.lo;
+    if (depth == 0) return;   // uniform over the workgroup
+    const int kf = top ? P.nz - 1 - P.hi : P.lo, step = top ? -1 : 1;
+    const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+    const int x = x0 + (int)threadIdx.x % T, y = y0 + (int)threadIdx.x / T;
+    const bool own = x < P.nx && y < P.ny;
+    auto at = [&](int xx, int yy, int k) { return ((long long)wrapc(xx, P.nx) * P.ny + wrapc(yy, P.ny)) * P.nz + k; };
+    if (own) {
+        const double m = muw[(long long)top * P.nx * P.ny + (long long)x * P.ny + y];
+        for (int d = 0; d < depth; ++d) mu[at(x, y, kf - (d + 1) * step)] = (TS)m;
+    }
+    if (!P.wall_scheme) {   // simple scheme: every layer from the first fluid layer
+        if (!own) return;
+        const double v = ((double)rho[at(x, y, kf)] * 2 + (double)rho[at(x - 1, y - 1, kf)] + (double)rho[at(x - 1, y + 1, kf)]
+                          + (double)rho[at(x + 1, y - 1, kf)] + (double)rho[at(x + 1, y + 1, kf)]) / 6;
+        for (int d = 1; d <= depth; ++d) rho[at(x, y, kf - d * step)] = (TS)v;
+        return;
+    }
+    for (int i = threadIdx.x; i < E * E; i += NT) buf[0][i] = (double)rho[at(x0 - R + i % E, y0 - R + i / E, kf)];
+    __syncthreads();
+    for (int d = 1; d <= depth; ++d) {
+        const double *cur = buf[(d - 1) & 1];
+        double *nxt = buf[d & 1];
+        // layer d is needed on the patch [d, E - d)^2
+        for (int i = threadIdx.x; i < E * E; i += NT) {
+            const int a = i % E, b = i / E;
+            if (a < d || a >= E - d || b < d || b >= E - d) continue;
+            nxt[i] = (double)(TS)((cur[i] * 2 + cur[i - 1 - E] + cur[i - 1 + E] + cur[i + 1 - E] + cur[i + 1 + E]) / 6);
+        }
+        __syncthreads();
+        if (own) rho[at(x, y, kf - d * step)] = (TS)nxt[(y - y0 + R) * E + (x - x0 + R)];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Chemical potential (ChemPotential, FluidCuda.cu:444-463):
 //   mu = K^2 mu_eos(clamp(rho)) - Ka * Laplacian(rho).
@@ -1032,9 +1080,15 @@ hipError_t launch_wall(cpmp_ctx *c) {
     const int passes = std::max(P.lo, P.hi);
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
-        for (int p = 0; p < passes; ++p)
-            hipLaunchKernelGGL(k_wall<TS>, dim3(blocks_for(c->NXY)), dim3(TPB), 0, c->stream, P, (TS *)c->rho,
-                               (TS *)c->mu, c->mu_w, p);
+        const dim3 g(blocks_for(c->NXY));
+        const dim3 gt((P.nx + kWallTile - 1) / kWallTile, (P.ny + kWallTile - 1) / kWallTile, 2), bt(kWallTile * kWallTile);
+        TS *rho = (TS *)c->rho, *mu = (TS *)c->mu;
+        if (passes == 1) hipLaunchKernelGGL((k_wall_tile<1, TS>), gt, bt, 0, c->stream, P, rho, mu, c->mu_w);
+        else if (passes == 2) hipLaunchKernelGGL((k_wall_tile<2, TS>), gt, bt, 0, c->stream, P, rho, mu, c->mu_w);
+        else if (passes == 3) hipLaunchKernelGGL((k_wall_tile<3, TS>), gt, bt, 0, c->stream, P, rho, mu, c->mu_w);
+        else   // deeper walls: one launch per layer
+            for (int p = 0; p < passes; ++p)
+                hipLaunchKernelGGL(k_wall<TS>, g, dim3(TPB), 0, c->stream, P, rho, mu, c->mu_w, p);
         return hipGetLastError();
     });
 }

@@ -30,7 +30,7 @@ EXPORTS = [
     "cpmp_set_wall_mu", "cpmp_step", "cpmp_sync", "cpmp_step_count", "cpmp_get_field",
     "cpmp_set_field", "cpmp_reduce", "cpmp_contact_angle", "cpmp_cap_fit", "cpmp_probe",
     "cpmp_laplace_jump",
-    "cpmp_checkpoint_save", "cpmp_checkpoint_load", "cpmp_write_plt", "cpmp_adjoint",
+    "cpmp_checkpoint_save", "cpmp_checkpoint_load", "cpmp_write_plt", "cpmp_adjoint", "cpmp_adjoint_plan",
     "cpmp_kernel_times", "cpmp_time_steps", "cpmp_algorithmic_bytes_per_node",
     "cpmp_last_error", "cpmp_version", "cpmp_selftest_collide",
 ]
@@ -62,7 +62,13 @@ class Stats(C.Structure):
 
 class AdjDesc(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("checkpoint_every", C.c_int),
-                ("target", C.POINTER(C.c_double)), ("tape_budget_bytes", C.c_int64)]
+                ("target", C.POINTER(C.c_double)), ("tape_budget_bytes", C.c_int64),
+                ("memory_limit_bytes", C.c_int64)]
+
+
+class AdjPlan(C.Structure):
+    _fields_ = [("device_bytes", C.c_int64), ("taped_steps", C.c_int), ("segments", C.c_int),
+                ("sub_length", C.c_int), ("sub_checkpoints", C.c_int), ("recomputed_steps", C.c_int64)]
 
 
 class CPMPError(RuntimeError):
@@ -111,6 +117,7 @@ def _load():
     L.cpmp_checkpoint_load.argtypes = [p, C.c_char_p]
     L.cpmp_write_plt.argtypes = [p, C.c_char_p, C.c_char_p, i, i]
     L.cpmp_adjoint.argtypes = [p, C.POINTER(AdjDesc), p, p, p, p]
+    L.cpmp_adjoint_plan.argtypes = [p, C.POINTER(AdjDesc), C.POINTER(AdjPlan)]
     L.cpmp_probe.argtypes = [p, i, i, i, i, p]
     L.cpmp_laplace_jump.argtypes = [p, p, p, p]
     L.cpmp_kernel_times.argtypes = [p, i, p, p]
@@ -309,21 +316,37 @@ class Lattice:
     def write_plt(self, path, title="Model3D", decimate=1, decimate_linear=False):
         check(lib.cpmp_write_plt(self.h, path.encode(), title.encode(), decimate, int(decimate_linear)))
 
-    def adjoint(self, nsteps, target, checkpoint_every=100, want_hi=True, tape_budget_bytes=0):
-        """cpmp_adjoint.  tape_budget_bytes: trailing steps taped in the first
-        forward run instead of recomputed (0: recompute everything, -1: half of
-        the free device memory)."""
-        nxy = self.p.nx * self.p.ny
-        t = np.ascontiguousarray(target, np.float64).ravel()
+    def _adj_desc(self, nsteps, checkpoint_every, tape_budget_bytes, memory_limit_bytes, target=None):
         d = AdjDesc()
         d.nsteps, d.checkpoint_every = nsteps, checkpoint_every
         d.tape_budget_bytes = int(tape_budget_bytes)
-        d.target = t.ctypes.data_as(C.POINTER(C.c_double))
+        d.memory_limit_bytes = int(memory_limit_bytes)
+        if target is not None:
+            d.target = target.ctypes.data_as(C.POINTER(C.c_double))
+        return d
+
+    def adjoint(self, nsteps, target, checkpoint_every=100, want_hi=True, tape_budget_bytes=0,
+                memory_limit_bytes=0):
+        """cpmp_adjoint.  tape_budget_bytes: trailing steps taped in the first
+        forward run instead of recomputed (0: recompute everything, -1: half of
+        the free device memory).  memory_limit_bytes: device bytes the call may
+        allocate (0: the free device memory less 1 GiB); segments whose tape does
+        not fit are recomputed in sub-segments."""
+        nxy = self.p.nx * self.p.ny
+        t = np.ascontiguousarray(target, np.float64).ravel()
+        d = self._adj_desc(nsteps, checkpoint_every, tape_budget_bytes, memory_limit_bytes, t)
         J = np.zeros(1)
         glo, ghi, gka = np.zeros(nxy), np.zeros(nxy), np.zeros(1)
         check(lib.cpmp_adjoint(self.h, C.byref(d), _ptr(J), _ptr(glo), _ptr(ghi) if want_hi else None,
                                _ptr(gka)))
         return J[0], glo, ghi, gka[0]
+
+    def adjoint_plan(self, nsteps, checkpoint_every=100, tape_budget_bytes=0, memory_limit_bytes=0):
+        """cpmp_adjoint_plan: the schedule and device bytes of an adjoint call"""
+        d = self._adj_desc(nsteps, checkpoint_every, tape_budget_bytes, memory_limit_bytes)
+        q = AdjPlan()
+        check(lib.cpmp_adjoint_plan(self.h, C.byref(d), C.byref(q)))
+        return {k: getattr(q, k) for k, _ in AdjPlan._fields_}
 
     def probe(self, field, x, y, z):
         """one node's rho or mu (cpmp_probe)"""

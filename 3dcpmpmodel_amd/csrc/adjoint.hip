@@ -15,13 +15,17 @@
 //   wall    : solid rho = layered diagonal averages of the first fluid layer (symmetric)
 // The forward trajectory is recomputed segment by segment from population
 // checkpoints; the tape holds rho (after the wall fill), u, grad mu and grad rho of
-// every step of the segment.  Trailing steps that fit the tape budget are taped during the first
-// forward run and not recomputed.
+// every step of the segment, in the storage scalar type of the form (the values the
+// forward collide used).  Trailing steps that fit the tape budget are taped during
+// the first forward run and not recomputed; segments whose tape does not fit the
+// memory limit are split into sub-segments with their own checkpoints.  The
+// reverse sweep runs in FP64 in every form.
 #include "internal.h"
 #include "lbm_math.h"
 #include "device_common.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace cpmp {
@@ -43,7 +47,8 @@ __device__ __forceinline__ void meq_vjp(double rho, double vx, double vy, double
 }
 
 // J and its seed: rb = rho - target on fluid nodes
-__global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const double *__restrict__ rho,
+template <class TT>
+__global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const TT *__restrict__ rho,
                                                const double *__restrict__ target, double *__restrict__ rb,
                                                double *__restrict__ part) {
     const long long I = (long long)blockIdx.x * TPB + threadIdx.x;
@@ -52,7 +57,7 @@ __global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const double *
         if (is_solid(P, (int)((unsigned long long)I % (unsigned)P.nz))) {
             rb[I] = 0;
         } else {
-            const double d = rho[I] - target[I];
+            const double d = (double)rho[I] - target[I];
             rb[I] = d;
             v = 0.5 * d * d;
         }
@@ -68,10 +73,11 @@ __global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const double *
 // without rbar is local to y and was written by the previous reverse step as fh_in
 // (fbar_q + ubar.(c_q - u)/rho), so each pulled value costs two loads.  This kernel
 // writes the same folded form for step n: fh_out_q = fbar_q + ubar.(c_q - u^n)/rho^n.
+template <class TT>
 __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const double *__restrict__ fh_in,
                                                      const double *__restrict__ rb_in,
-                                                     const double *__restrict__ rho0, const double *__restrict__ u0,
-                                                     const double *__restrict__ grads, double *__restrict__ fh_out,
+                                                     const TT *__restrict__ rho0, const TT *__restrict__ u0,
+                                                     const TT *__restrict__ grads, double *__restrict__ fh_out,
                                                      double *__restrict__ rb_loc,
                                                      double *__restrict__ Pf, double *__restrict__ Wf) {
     const NodeIdx nd = node_from_grid(P);
@@ -108,10 +114,10 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
     double fbar[Q];
     inverse_scaled(b, fbar);
 
-    const double rho = rho0[I], ux = u0[I], uy = u0[N + I], uz = u0[2 * N + I];
+    const double rho = (double)rho0[I], ux = (double)u0[I], uy = (double)u0[N + I], uz = (double)u0[2 * N + I];
     // grad mu^n and grad rho^n as the forward collide evaluated them (taped)
-    const double gmx = grads[I], gmy = grads[N + I], gmz = grads[2 * N + I];
-    const double grx = grads[3 * N + I], gry = grads[4 * N + I], grz = grads[5 * N + I];
+    const double gmx = (double)grads[I], gmy = (double)grads[N + I], gmz = (double)grads[2 * N + I];
+    const double grx = (double)grads[3 * N + I], gry = (double)grads[4 * N + I], grz = (double)grads[5 * N + I];
     const double Fx = -gmx * rho + grx / 3, Fy = -gmy * rho + gry / 3, Fz = -gmz * rho + grz / 3;
     const double ex = ux + Fx / rho, ey = uy + Fy / rho, ez = uz + Fz / rho;
     double wu[16];
@@ -162,7 +168,8 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const do
 
 // transposed chemical potential: Laplacian of the fluid mu cotangent, EOS slope,
 // and the kappa gradient (per-block partial sums)
-__global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const double *__restrict__ rho0,
+template <class TT>
+__global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const TT *__restrict__ rho0,
                                                 const double *__restrict__ mub, double *__restrict__ rb_new,
                                                 double *__restrict__ kapart) {
     const NodeIdx nd = node_from_grid(P);
@@ -174,7 +181,7 @@ __global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const double 
         wrap7(P, c, xs, ys, zs);
         auto idx = [&](int dx, int dy, int dz) { return ((long long)xs[dx + 3] * P.ny + ys[dy + 3]) * P.nz + zs[dz + 3]; };
         auto mm = [&](int dx, int dy, int dz) { return is_solid(P, zs[dz + 3]) ? 0.0 : mub[idx(dx, dy, dz)]; };
-        auto rr = [&](int dx, int dy, int dz) { return rho0[idx(dx, dy, dz)]; };
+        auto rr = [&](int dx, int dy, int dz) { return (double)rho0[idx(dx, dy, dz)]; };
         auto lap = [&](auto &&v) {
             if (P.lap_points == 7) {
                 auto ax = [&](int ux, int uy, int uz) {
@@ -190,7 +197,7 @@ __global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const double 
         };
         double acc = -P.Ka * lap(mm);
         if (!is_solid(P, c.z)) {
-            const double r = rho0[I], mu_b = mub[I];
+            const double r = (double)rho0[I], mu_b = mub[I];
             acc += P.K2s * dmu_eos(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho(r)) * clamp_drho(r) * mu_b;
             kap = -mu_b * lap(rr);
         }
@@ -315,13 +322,117 @@ using namespace cpmp;
         if (e_ != hipSuccess) { rc = set_error(CPMP_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); goto done; } \
     } while (0)
 
+namespace {
+
+// Memory plan of one adjoint call.  The trailing ntaped steps are taped during the
+// first forward run; the n0 steps before them are covered by checkpoints every C
+// steps and recomputed segment by segment with a working tape of S entries.  When
+// S < C a segment is recomputed twice: a first pass stores sub-checkpoints every
+// S steps (and tapes the last sub-segment), then every earlier sub-segment is
+// restored and taped in turn.
+struct AdjPlan {
+    int C = 0, ntaped = 0, n0 = 0, nseg = 0, S = 0, nsubck = 0, tape_len = 1;
+    size_t per_ck = 0;       // bytes of one checkpoint: populations + rho in the storage form
+    size_t per_tape = 0;     // bytes of one tape entry: 10 values of the storage scalar type per node
+    size_t nbuf = 0;         // doubles of the cotangent and work buffers
+    size_t fixed = 0;        // bytes besides checkpoints and tape
+    size_t total = 0;        // device bytes allocated by the call
+    long long recomputed = 0;
+};
+
+int sub_count(int L, int S) { return (L + S - 1) / S; }
+
+size_t plan_total(const AdjPlan &q, int S, int *nsubck, int *tape_len) {
+    const int nsub = q.n0 > 0 ? sub_count(std::min(q.C, q.n0), S) : 0;
+    *nsubck = std::max(nsub - 2, 0);
+    *tape_len = std::max(std::max(q.ntaped, q.n0 > 0 ? S : 0), 1);
+    return (size_t)std::max(q.nseg, 1) * q.per_ck + (size_t)*nsubck * q.per_ck + (size_t)*tape_len * q.per_tape + q.fixed;
+}
+
+int make_plan(cpmp_ctx *c, const cpmp_adj_desc *d, AdjPlan &q) {
+    if (c->form != CPMP_FORM_FP64 && c->form != CPMP_FORM_FP32)
+        return set_error(CPMP_EINVAL, "the adjoint runs in the FP64 and FP32 forms");
+    if (d->nsteps < 1 || d->checkpoint_every < 1) return set_error(CPMP_EINVAL, "nsteps and checkpoint_every must be >= 1");
+    const long long N = c->N, NXY = c->NXY;
+    const dim3 ng = node_grid(c->dp);
+    const long long nb = blocks_for(N), nbg = (long long)ng.x * ng.y * ng.z;
+    q.C = std::min(d->checkpoint_every, d->nsteps);
+    q.per_ck = (size_t)N * (19 * c->pop_bytes + c->scal_bytes);
+    q.per_tape = 10 * (size_t)N * c->scal_bytes;
+    q.nbuf = (size_t)N * (19 * 2 + 3 + 6 + 1) + std::max(nb, nbg) + nbg + 2 * NXY + 4;
+    // buffers, the final state in the storage form, the target
+    q.fixed = q.nbuf * sizeof(double) + q.per_ck + (size_t)N * sizeof(double);
+    size_t fr = 0, tot = 0;
+    const bool need_free = d->tape_budget_bytes < 0 || d->memory_limit_bytes == 0;
+    if (need_free) {
+        const hipError_t e = hipMemGetInfo(&fr, &tot);
+        if (e != hipSuccess) return set_error(CPMP_EHIP, "hipMemGetInfo: %s", hipGetErrorString(e));
+    }
+    long long budget = d->tape_budget_bytes < 0 ? (long long)(fr / 2) : d->tape_budget_bytes;
+    q.ntaped = (int)std::min<long long>(budget / (long long)q.per_tape, d->nsteps);
+    q.n0 = d->nsteps - q.ntaped;
+    q.nseg = q.n0 > 0 ? (q.n0 + q.C - 1) / q.C : 0;
+    // memory limit: given, or the free device memory less 2 GiB
+    const size_t margin = (size_t)2 << 30;
+    const size_t limit = d->memory_limit_bytes > 0 ? (size_t)d->memory_limit_bytes : (fr > margin ? fr - margin : 0);
+    // forward steps run during the reverse sweep with sub-segments of S steps: a
+    // segment of L steps costs L + (ceil(L / S) - 1) S (the last sub-segment is
+    // taped in the first pass)
+    const int Lend = q.nseg > 0 ? q.n0 - (q.nseg - 1) * q.C : 0;
+    auto recompute = [&](int S) {
+        auto seg = [&](long long L) { return L + (long long)(sub_count((int)L, S) - 1) * S; };
+        return q.nseg > 0 ? (long long)(q.nseg - 1) * seg(q.C) + seg(Lend) : 0LL;
+    };
+    // the schedule with the fewest recomputed steps that fits, then the least memory
+    int best = -1;
+    long long best_rc = 0;
+    size_t best_total = 0, least = SIZE_MAX;
+    for (int S = q.n0 > 0 ? std::min(q.C, q.n0) : 0; S >= (q.n0 > 0 ? 1 : 0); --S) {
+        int nsc, tl;
+        const size_t total = plan_total(q, S, &nsc, &tl);
+        least = std::min(least, total);
+        if (total > limit) continue;
+        const long long rcs = recompute(S);
+        if (best < 0 || rcs < best_rc || (rcs == best_rc && total < best_total)) {
+            best = S;
+            best_rc = rcs;
+            best_total = total;
+        }
+        if (q.n0 == 0) break;
+    }
+    if (best < 0)
+        return set_error(CPMP_ENOMEM, "adjoint needs at least %zu bytes of device memory, limit %zu "
+                         "(fewer checkpoints or a smaller tape budget)", least, limit);
+    q.S = best;
+    q.total = plan_total(q, best, &q.nsubck, &q.tape_len);
+    q.recomputed = best_rc;
+    return CPMP_OK;
+}
+
+}  // namespace
+
+extern "C" int cpmp_adjoint_plan(cpmp_ctx *c, const cpmp_adj_desc *d, cpmp_adj_plan *out) {
+    if (!c || !c->stream || !d || !out) return set_error(CPMP_EINVAL, "bad arguments");
+    if (hipSetDevice(c->p.device) != hipSuccess) return set_error(CPMP_EHIP, "hipSetDevice");
+    AdjPlan q;
+    const int rc = make_plan(c, d, q);
+    if (rc != CPMP_OK) return rc;
+    out->device_bytes = (int64_t)q.total;
+    out->taped_steps = q.ntaped;
+    out->segments = q.nseg;
+    out->sub_length = q.S;
+    out->sub_checkpoints = q.nsubck;
+    out->recomputed_steps = q.recomputed;
+    return CPMP_OK;
+}
+
 extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, double *g_lo, double *g_hi, double *g_ka) {
     if (!c || !c->stream || !d || !d->target || !J) return set_error(CPMP_EINVAL, "bad arguments");
-    if (c->form != CPMP_FORM_FP64) return set_error(CPMP_EINVAL, "the adjoint runs in the FP64 form only");
-    if (d->nsteps < 1 || d->checkpoint_every < 1) return set_error(CPMP_EINVAL, "nsteps and checkpoint_every must be >= 1");
-    int rc = CPMP_OK;
+    if (hipSetDevice(c->p.device) != hipSuccess) return set_error(CPMP_EHIP, "hipSetDevice");
+    AdjPlan q;
+    int rc = make_plan(c, d, q);
+    if (rc != CPMP_OK) return rc;
     const long long N = c->N, NXY = c->NXY;
-    const int C = std::min(d->checkpoint_every, d->nsteps);
     const unsigned nb = blocks_for(N), nbc = blocks_for(NXY);
     const dim3 ng = node_grid(c->dp);
     const long long nbg = (long long)ng.x * ng.y * ng.z;   // blocks of the node grid
@@ -330,36 +441,21 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const dim3 gwt((P.nx + kWallTile - 1) / kWallTile, (P.ny + kWallTile - 1) / kWallTile, 2), bwt(kWallTile * kWallTile);
     const int passes = std::max(P.lo, P.hi);
     const int64_t step0 = c->step;
-    double *ckpt = nullptr, *tape = nullptr, *buf = nullptr, *tgt = nullptr;
-    double *fbA, *fbB, *rb, *rbn, *rbl, *Pf, *Wf, *mub, *part, *kpart, *gl, *gh, *acc, *fin_f, *fin_r;
-    const size_t per_ck = 20 * (size_t)N;          // f (19N) + rho (N)
-    // tape entry of step n: rho^n after the wall fill (N), then u^n (3N), grad mu^n
-    // (3N) and grad rho^n (3N) as the forward collide evaluated them
-    const size_t per_tape = 10 * (size_t)N;
-    const size_t nbuf = (size_t)N * (19 * 2 + 1 + 1 + 1 + 3 + 3 + 1) + std::max<long long>(nb, nbg) + nbg + 2 * NXY + 4 + 20 * (size_t)N;
+    const size_t fbytes = 19 * (size_t)N * c->pop_bytes, rbytes = (size_t)N * c->scal_bytes;
+    // tape entry of step n, in the storage scalar type (FP64 or FP32): rho^n after
+    // the wall fill (N), then u^n (3N), grad mu^n (3N) and grad rho^n (3N) as the
+    // forward collide evaluated them
+    const size_t sb = c->scal_bytes;
+    char *ckpt = nullptr, *subck = nullptr, *fin = nullptr, *tape = nullptr;
+    double *buf = nullptr, *tgt = nullptr;
+    double *fbA, *fbB, *rb, *rbn, *rbl, *Pf, *Wf, *mub, *part, *kpart, *gl, *gh, *acc;
     double acc_h[2] = {0, 0};
-
-    AHIP(hipSetDevice(c->p.device));
     {
-        // trailing steps taped during the first forward run (no recompute); the
-        // rest is covered by checkpoints every C steps and recomputed per segment
-        long long budget = d->tape_budget_bytes;
-        if (budget < 0) {
-            size_t fr = 0, tot = 0;
-            AHIP(hipMemGetInfo(&fr, &tot));
-            budget = (long long)(fr / 2);
-        }
-        const long long per_step = (long long)(per_tape * sizeof(double));
-        long long T = budget / per_step;
-        T = std::min<long long>(T, d->nsteps);
-        const int ntaped = (int)T;
-        const int n0 = d->nsteps - ntaped;              // steps covered by checkpoints
-        const int nseg = n0 > 0 ? (n0 + C - 1) / C : 0;
-        const int tape_len = std::max(std::max(ntaped, n0 > 0 ? C : 0), 1);
-
-        AHIP(hipMalloc((void **)&ckpt, (size_t)std::max(nseg, 1) * per_ck * sizeof(double)));
-        AHIP(hipMalloc((void **)&tape, (size_t)tape_len * per_tape * sizeof(double)));
-        AHIP(hipMalloc((void **)&buf, nbuf * sizeof(double)));
+        AHIP(hipMalloc((void **)&ckpt, (size_t)std::max(q.nseg, 1) * q.per_ck));
+        if (q.nsubck > 0) AHIP(hipMalloc((void **)&subck, (size_t)q.nsubck * q.per_ck));
+        AHIP(hipMalloc((void **)&fin, q.per_ck));
+        AHIP(hipMalloc((void **)&tape, (size_t)q.tape_len * q.per_tape));
+        AHIP(hipMalloc((void **)&buf, q.nbuf * sizeof(double)));
         AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
         {
             double *p = buf;
@@ -368,29 +464,49 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
             part = p; p += std::max<long long>(nb, nbg); kpart = p; p += nbg;
             gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
-            fin_f = p; p += 19 * N; fin_r = p; p += N;
         }
         AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
-        AHIP(hipMemsetAsync(buf, 0, nbuf * sizeof(double), s));
+        AHIP(hipMemsetAsync(buf, 0, q.nbuf * sizeof(double), s));
 
+        // state = populations of the current set + rho, in the storage form
+        auto save = [&](char *dst) -> hipError_t {
+            hipError_t e = hipMemcpyAsync(dst, c->f[c->cur], fbytes, hipMemcpyDeviceToDevice, s);
+            return e != hipSuccess ? e : hipMemcpyAsync(dst + fbytes, c->rho, rbytes, hipMemcpyDeviceToDevice, s);
+        };
+        auto restore = [&](const char *src) -> hipError_t {
+            hipError_t e = hipMemcpyAsync(c->f[c->cur], src, fbytes, hipMemcpyDeviceToDevice, s);
+            return e != hipSuccess ? e : hipMemcpyAsync(c->rho, src + fbytes, rbytes, hipMemcpyDeviceToDevice, s);
+        };
         // one forward step that records its tape entry tr
-        auto taped_step = [&](double *tr) -> hipError_t {
+        auto taped_step = [&](char *tr) -> hipError_t {
             hipError_t e;
             if ((e = launch_wall(c)) != hipSuccess) return e;
-            if ((e = hipMemcpyAsync(tr, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+            if ((e = hipMemcpyAsync(tr, c->rho, N * sb, hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
             if ((e = launch_mu(c)) != hipSuccess) return e;
-            if ((e = launch_collide(c, false, tr + N)) != hipSuccess) return e;
+            if ((e = launch_collide(c, false, tr + N * sb)) != hipSuccess) return e;
             return launch_macro(c);
+        };
+        auto tape_steps = [&](int L) -> hipError_t {
+            for (int k = 0; k < L; ++k) {
+                const hipError_t e = taped_step(tape + k * q.per_tape);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
         };
         // reverse sweep over L taped steps (entries 0..L-1)
         auto reverse = [&](int L) -> hipError_t {
             for (int k = L - 1; k >= 0; --k) {
-                const double *t0 = tape + k * per_tape;
-                hipLaunchKernelGGL(k_adj_collide, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA,
-                                   (const double *)rb, t0, t0 + N, t0 + 4 * N, fbB, rbl, Pf, Wf);
-                hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf, (const double *)Wf,
-                                   (const double *)rbl, mub, rbn);
-                hipLaunchKernelGGL(k_adj_mu, node_grid(P), dim3(TPB), 0, s, P, t0, (const double *)mub, rbn, kpart);
+                const char *t0 = tape + k * q.per_tape;
+                auto tape_kernels = [&](auto *tt) {
+                    using TT = std::remove_const_t<std::remove_pointer_t<decltype(tt)>>;
+                    hipLaunchKernelGGL(k_adj_collide<TT>, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA,
+                                       (const double *)rb, tt, tt + N, tt + 4 * N, fbB, rbl, Pf, Wf);
+                    hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf,
+                                       (const double *)Wf, (const double *)rbl, mub, rbn);
+                    hipLaunchKernelGGL(k_adj_mu<TT>, node_grid(P), dim3(TPB), 0, s, P, tt, (const double *)mub, rbn, kpart);
+                };
+                if (sb == sizeof(double)) tape_kernels((const double *)t0);
+                else tape_kernels((const float *)t0);
                 if (passes == 1)
                     hipLaunchKernelGGL(k_adj_wall_tile<1>, gwt, bwt, 0, s, P, rbn, (const double *)mub, gl, gh);
                 else if (passes == 2)
@@ -410,30 +526,45 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         };
 
         // ---- forward: checkpoints over the first n0 steps, then taped steps ----
-        for (int sg = 0; sg < nseg; ++sg) {
-            double *ck = ckpt + sg * per_ck;
-            AHIP(hipMemcpyAsync(ck, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
-            AHIP(hipMemcpyAsync(ck + 19 * N, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-            const int L = std::min(C, n0 - sg * C);
+        for (int sg = 0; sg < q.nseg; ++sg) {
+            AHIP(save(ckpt + sg * q.per_ck));
+            const int L = std::min(q.C, q.n0 - sg * q.C);
             for (int k = 0; k < L; ++k) AHIP(launch_step(c, false));
         }
-        for (int k = 0; k < ntaped; ++k) AHIP(taped_step(tape + k * per_tape));
-        AHIP(hipMemcpyAsync(fin_f, c->f[c->cur], 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
-        AHIP(hipMemcpyAsync(fin_r, c->rho, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_adj_J, dim3(nb), dim3(TPB), 0, s, P, (const double *)c->rho, (const double *)tgt, rb, part);
+        AHIP(tape_steps(q.ntaped));
+        AHIP(save(fin));
+        if (sb == sizeof(double))
+            hipLaunchKernelGGL(k_adj_J<double>, dim3(nb), dim3(TPB), 0, s, P, (const double *)c->rho, (const double *)tgt, rb, part);
+        else
+            hipLaunchKernelGGL(k_adj_J<float>, dim3(nb), dim3(TPB), 0, s, P, (const float *)c->rho, (const double *)tgt, rb, part);
         AHIP(hipGetLastError());
         hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)part, (long long)nb, acc);
         AHIP(hipGetLastError());
 
         // ---- reverse: the taped steps, then the checkpoint segments ----
-        if (ntaped > 0) AHIP(reverse(ntaped));
-        for (int sg = nseg - 1; sg >= 0; --sg) {
-            const double *ck = ckpt + sg * per_ck;
-            AHIP(hipMemcpyAsync(c->f[c->cur], ck, 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
-            AHIP(hipMemcpyAsync(c->rho, ck + 19 * N, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-            const int L = std::min(C, n0 - sg * C);
-            for (int k = 0; k < L; ++k) AHIP(taped_step(tape + k * per_tape));
-            AHIP(reverse(L));
+        if (q.ntaped > 0) AHIP(reverse(q.ntaped));
+        for (int sg = q.nseg - 1; sg >= 0; --sg) {
+            const char *ck = ckpt + sg * q.per_ck;
+            const int L = std::min(q.C, q.n0 - sg * q.C), S = q.S, nsub = sub_count(L, S);
+            AHIP(restore(ck));
+            if (nsub == 1) {
+                AHIP(tape_steps(L));
+                AHIP(reverse(L));
+                continue;
+            }
+            // first pass: sub-checkpoints at S, 2S, .. (nsub-2)S, last sub-segment taped
+            for (int j = 1; j < nsub; ++j) {
+                for (int k = 0; k < S; ++k) AHIP(launch_step(c, false));
+                if (j <= nsub - 2) AHIP(save(subck + (j - 1) * q.per_ck));
+            }
+            const int Llast = L - (nsub - 1) * S;
+            AHIP(tape_steps(Llast));
+            AHIP(reverse(Llast));
+            for (int j = nsub - 2; j >= 0; --j) {
+                AHIP(restore(j == 0 ? ck : subck + (j - 1) * q.per_ck));
+                AHIP(tape_steps(S));
+                AHIP(reverse(S));
+            }
         }
     }
     hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)kpart, nbg, acc + 1);
@@ -442,17 +573,22 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     if (g_lo) AHIP(hipMemcpyAsync(g_lo, gl, NXY * sizeof(double), hipMemcpyDeviceToHost, s));
     if (g_hi) AHIP(hipMemcpyAsync(g_hi, gh, NXY * sizeof(double), hipMemcpyDeviceToHost, s));
     // leave the context at the end of the forward run
-    AHIP(hipMemcpyAsync(c->f[c->cur], fin_f, 19 * N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    AHIP(hipMemcpyAsync(c->rho, fin_r, N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    {
+        const hipError_t e = hipMemcpyAsync(c->f[c->cur], fin, fbytes, hipMemcpyDeviceToDevice, s);
+        AHIP(e);
+    }
+    AHIP(hipMemcpyAsync(c->rho, fin + fbytes, rbytes, hipMemcpyDeviceToDevice, s));
     AHIP(hipStreamSynchronize(s));
     *J = acc_h[0];
     if (g_ka) *g_ka = acc_h[1];
     c->step = step0 + d->nsteps;
     c->mv_valid = false;
 done:
-    if (ckpt) hipFree(ckpt);
-    if (tape) hipFree(tape);
-    if (buf) hipFree(buf);
-    if (tgt) hipFree(tgt);
+    if (ckpt) (void)hipFree(ckpt);
+    if (subck) (void)hipFree(subck);
+    if (fin) (void)hipFree(fin);
+    if (tape) (void)hipFree(tape);
+    if (buf) (void)hipFree(buf);
+    if (tgt) (void)hipFree(tgt);
     return rc;
 }

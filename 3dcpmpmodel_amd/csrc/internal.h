@@ -76,8 +76,9 @@ constexpr int kTileY = 8, kTileZ = 32;   // in-plane tile of the march kernels a
 constexpr int kWallTile = 16;             // columns per edge of the wall-fill tiles (16 x 16 workgroups)
 // copy one population set between the device layout and the reference node order
 hipError_t launch_pop_relayout(cpmp_ctx *c, const void *src, void *dst, bool to_device_layout);
-// uout (when given): u, grad mu, grad rho of every node, 9 N doubles (adjoint tape)
-hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout = nullptr);
+// uout (when given): u, grad mu, grad rho of every node, 9 N values of the
+// storage scalar type (adjoint tape)
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout = nullptr);
 hipError_t launch_macro(cpmp_ctx *c);
 hipError_t launch_step(cpmp_ctx *c, bool want_mv);
 hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, double rg, double rl, double solid);

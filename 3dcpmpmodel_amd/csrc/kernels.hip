@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restr
 template <class TF, class TS, class TC, bool MV>
 __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__restrict__ fin, TF *__restrict__ fout,
                                                  const TS *__restrict__ rho, const TS *__restrict__ mu,
-                                                 double *__restrict__ mv, double *__restrict__ uout,
+                                                 double *__restrict__ mv, TS *__restrict__ uout,
                                                  const TS *__restrict__ rref) {
     const NodeIdx nd = node_from_grid(P);
     if (!nd.ok) return;
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
     if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
         const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
 #pragma unroll
-        for (int a = 0; a < 9; ++a) uout[a * N + I] = (double)t9[a];
+        for (int a = 0; a < 9; ++a) uout[a * N + I] = (TS)t9[a];
     }
     if (MV) {
         const TC h = TC(0.5) / r;
@@ -433,7 +433,7 @@ template <class TF, class TS, class TC, bool MV, bool RS>
 __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
-                                                       double *__restrict__ uout, int ntz, int nty,
+                                                       TS *__restrict__ uout, int ntz, int nty,
                                                        TS *__restrict__ rho_next, TS *__restrict__ rbuf,
                                                        TS *__restrict__ cbuf, int remap, const TS *__restrict__ rref) {
     __shared__ TS s_rho[7][MLP];
@@ -590,7 +590,7 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
             if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
                 const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
 #pragma unroll
-                for (int a = 0; a < 9; ++a) uout[a * N + I] = (double)t9[a];
+                for (int a = 0; a < 9; ++a) uout[a * N + I] = (TS)t9[a];
             }
             if (MV) {
                 const TC h = TC(0.5) / r;
@@ -1160,7 +1160,7 @@ hipError_t launch_mu(cpmp_ctx *c) {
     });
 }
 
-hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v) {
     const bool rs = density_fused(c);
     const hipError_t e = dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
@@ -1170,6 +1170,7 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, double *uout) {
         TF *fout = (TF *)c->f[c->cur ^ 1];
         const TS *rho = (const TS *)c->rho, *mu = (const TS *)c->mu;
         TS *rn = (TS *)c->rho_next, *rb = (TS *)c->rbuf, *cb = (TS *)c->cbuf;
+        TS *uout = (TS *)uout_v;
         // Tiles go to the XCDs in plain round-robin order: with tile-contiguous
         // populations the eight concurrent z tiles of a row then stream one
         // contiguous 8 KB run per population (measured faster than giving each

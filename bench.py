@@ -162,6 +162,104 @@ def adjoint_cost(cpmp, nsteps=2000, every=100):
             "J": res["recompute"][1], "J_taped": res["taped"][1]}
 
 
+def calibrate_wall_mu(cpmp, eos, tr, rho_g, rho_l, angle, mu_b, steps=6000, iters=12):
+    """Wall chemical potential giving a target contact angle: bisection on the
+    3-D cap-fit angle of a hemisphere of radius 16 on the bottom wall of a
+    64x64x48 lattice after `steps` FP64 steps (the angle grows with mu_w).  The
+    bracket starts at mu_b -/+ 0.05 (mu_b: the bulk chemical potential at
+    coexistence, a neutral wall) and widens until it holds the angle."""
+    nx, ny, nz = 64, 64, 48
+
+    def measure(mu_w):
+        g = cpmp.Lattice(cpmp.make_params(nx, ny, nz, eos=eos, tr=tr))
+        g.set_wall_mu(mu_w, mu_b)
+        g.init_spheres([(nx / 2, ny / 2, 2.5, 16.0)], 5.0, rho_g, rho_l, solid_rho=0.0)
+        g.step(steps)
+        a = g.cap_fit(rho_g, rho_l)["angle"]
+        g.close()
+        return a
+
+    lo, hi, w = mu_b - 0.05, mu_b + 0.05, 0.05
+    for _ in range(6):
+        if measure(lo) > angle:
+            lo -= 2 * w
+        elif measure(hi) < angle:
+            hi += 2 * w
+        else:
+            break
+        w *= 2
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        if measure(mid) < angle:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def large_run(cpmp, fwd_steps=10000, adj_steps=2000, every=100):
+    """Large FP32 wetting run with adjoint: 512x512x256 van der Waals fluid at
+    Tr = 0.8, bottom wall striped along x (period 64: 32 columns at the wall
+    chemical potential of 60 degrees, 32 at that of 120 degrees, both calibrated
+    with calibrate_wall_mu), top wall neutral, hemisphere of radius 100.  The
+    forward run takes fwd_steps steps; the adjoint covers the last adj_steps with
+    checkpoints every `every` steps (20 checkpoints), objective the mismatch of
+    the final density against the density at the start of the adjoint window.
+    Reports MLUPS, roofline fraction, adjoint/forward time and device memory."""
+    import numpy as np
+    eos, tr = "VDW", 0.8
+    A, B, Tc, Rc = cpmp.set_multiphase(eos, tr)
+    rho_g, rho_l = cpmp.coexistence(eos, A, B, tr * Tc)
+    p0 = cpmp.make_params(8, 8, 8, eos=eos, tr=tr)
+    mu_b = p0.K ** 2 * cpmp.eos_mu(eos, A, B, tr * Tc, rho_l)
+    t0 = time.perf_counter()
+    mu60 = calibrate_wall_mu(cpmp, eos, tr, rho_g, rho_l, 60.0, mu_b)
+    mu120 = calibrate_wall_mu(cpmp, eos, tr, rho_g, rho_l, 120.0, mu_b)
+    log(f"[large] calibrated wall mu: 60 deg {mu60:.6f}, 120 deg {mu120:.6f} (neutral {mu_b:.6f}), "
+        f"{time.perf_counter() - t0:.1f} s")
+    nx, ny, nz = 512, 512, 256
+    nodes = nx * ny * nz
+    lat = cpmp.Lattice(cpmp.make_params(nx, ny, nz, eos=eos, tr=tr, form="fp32"))
+    xs = np.repeat(np.arange(nx), ny)
+    lat.set_wall_mu(np.where((xs // 32) % 2 == 0, mu60, mu120), mu_b)
+    lat.init_spheres([(nx / 2, ny / 2, 2.5, 100.0)], 5.0, rho_g, rho_l, solid_rho=0.0)
+    n_plain = fwd_steps - adj_steps
+    kt_steps = 20
+    log(f"[large] forward {n_plain} steps ...")
+    t0 = time.perf_counter()
+    lat.step(n_plain - kt_steps)
+    t_plain = time.perf_counter() - t0
+    ktimes, _ = lat.kernel_times(kt_steps)
+    mlups = nodes * (n_plain - kt_steps) / t_plain / 1e6
+    bpn = lat.algorithmic_bytes_per_node()
+    s = lat.reduce()
+    log(f"[large] {mlups:.1f} MLUPS, kernels {ktimes}, mass {s.mass:.6e}")
+    target = lat.get("rho")
+    ctx_bytes = lat.memory()
+    plan = lat.adjoint_plan(adj_steps, every)
+    log(f"[large] adjoint plan {plan}, context {ctx_bytes / 2**30:.1f} GiB")
+    t0 = time.perf_counter()
+    J, glo, ghi, gka = lat.adjoint(adj_steps, target, checkpoint_every=every)
+    t_adj = time.perf_counter() - t0
+    lat.close()
+    t_fwd = t_plain / (n_plain - kt_steps) * adj_steps
+    kc = ktimes["collide"]
+    out = {"config": f"{nx}x{ny}x{nz} VDW Tr=0.8 FP32, stripes of period 64 (60/120 deg), hemisphere R=100, "
+                     f"{fwd_steps} steps, adjoint over the last {adj_steps} with checkpoints every {every}",
+           "mu_w_60": mu60, "mu_w_120": mu120, "mu_neutral": mu_b,
+           "mlups": mlups, "ms_per_step": t_plain / (n_plain - kt_steps) * 1e3,
+           "kernel_ms": ktimes,
+           "collide_roofline_frac": bpn * nodes / (kc * 1e-3) / 1e9 / PEAK_HBM_GBS,
+           "step_roofline_frac": mlups * 1e6 * bpn / 1e9 / PEAK_HBM_GBS,
+           "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+           "J": J, "grad_norm": float(np.linalg.norm(glo)), "dJ_dKa": gka,
+           "context_bytes": ctx_bytes, "adjoint_plan": plan,
+           "device_bytes_high_water": ctx_bytes + plan["device_bytes"]}
+    log(f"[large] adjoint {t_adj:.1f} s over forward {t_fwd:.1f} s = {t_adj / t_fwd:.2f}; "
+        f"device memory {(ctx_bytes + plan['device_bytes']) / 2**30:.1f} GiB")
+    return out
+
+
 def cpu_baseline(n, steps):
     """The FP64 OpenMP oracle (CPU restatement of the reference step) on the host
     cores, on a bounded sample: the same 256^3 workload for `steps` steps."""
@@ -198,6 +296,8 @@ def main():
     ap.add_argument("--forms", default="fp32,fp64,fp16")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--sessile-steps", type=int, default=4000)
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 512x512x256 FP32 wetting run with adjoint (about 2 minutes, ~270 GB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -230,6 +330,9 @@ def main():
         adj = adjoint_cost(cpmp)
         log(f"[bench] adjoint/forward = {adj['adjoint_over_forward']:.2f} (checkpoints + recompute), "
             f"{adj['adjoint_taped_over_forward']:.2f} (tape in device memory)")
+    large = None
+    if args.large:
+        large = large_run(cpmp)
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
@@ -265,6 +368,7 @@ def main():
         "adjoint_taped_over_forward": round(adj["adjoint_taped_over_forward"], 3) if adj else None,
         "accuracy": acc,
         "adjoint": adj,
+        "large": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

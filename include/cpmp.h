@@ -117,7 +117,23 @@ typedef struct {
        checkpoint_every steps and recomputed segment by segment.  0: checkpoints and
        recompute for every step; < 0: half of the free device memory. */
     int64_t tape_budget_bytes;
+    /* Limit on the device bytes the call allocates (0: the free device memory less
+       1 GiB).  A segment whose tape does not fit is recomputed in sub-segments:
+       a first pass stores sub-checkpoints, then each sub-segment is taped in turn
+       (each of its steps is recomputed twice).  The result does not depend on the
+       schedule. */
+    int64_t memory_limit_bytes;
 } cpmp_adj_desc;
+
+/* The schedule and memory of one adjoint call (cpmp_adjoint_plan). */
+typedef struct {
+    int64_t device_bytes;      /* device memory the call allocates besides the context */
+    int taped_steps;           /* trailing steps taped during the first forward run */
+    int segments;              /* checkpoint segments before them */
+    int sub_length;            /* steps per taped sub-segment (= segment length when it fits) */
+    int sub_checkpoints;       /* extra states stored per segment for the sub-segments */
+    int64_t recomputed_steps;  /* forward steps run again during the reverse sweep */
+} cpmp_adj_plan;
 
 typedef struct cpmp_ctx cpmp_ctx;
 
@@ -209,9 +225,15 @@ int cpmp_write_plt(cpmp_ctx *ctx, const char *path, const char *title, int decim
 /* ---- discrete adjoint (no reference counterpart) ---- */
 /* J = 1/2 sum_fluid (rho^N - target)^2 after d->nsteps steps from the current state.
    Returns dJ/d mu_w(bottom), dJ/d mu_w(top) (nx*ny each, may be NULL) and dJ/dKa.
-   The state is left at step N of the forward run.  FP64 form only. */
+   The state is left at step N of the forward run.  FP64 and FP32 forms: the forward
+   run (and its recomputation) is the context's own step; the reverse sweep runs in
+   FP64 on the tape, so in the FP32 form it is the FP64 adjoint evaluated along the
+   FP32 trajectory. */
 int cpmp_adjoint(cpmp_ctx *ctx, const cpmp_adj_desc *d, double *J, double *g_mu_lo,
                  double *g_mu_hi, double *g_ka);
+/* The schedule cpmp_adjoint would take for d, without running it (CPMP_ENOMEM when
+   no schedule fits the memory limit). */
+int cpmp_adjoint_plan(cpmp_ctx *ctx, const cpmp_adj_desc *d, cpmp_adj_plan *plan);
 
 /* ---- measurement ---- */
 /* Average device time per launch (ms) of each kernel over nsteps steps, measured

@@ -231,3 +231,89 @@ def test_adjoint_tape_budget_schedules_agree(cpmp):
         assert np.array_equal(r[1], res[0][1])
         assert np.array_equal(r[2], res[0][2])
         assert r[3] == res[0][3]
+
+
+def _small_lattice(cpmp, dims, form="fp64"):
+    kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims)
+    g = cpmp.Lattice(cpmp.make_params(*dims, form=form, **kw))
+    g.set_wall_mu(mu_lo, mu_hi)
+    g.init_from_rho(rho0)
+    return g, target
+
+
+def test_adjoint_memory_limit_sub_segments_agree(cpmp):
+    """a memory limit that the segment tape does not fit: the segment is
+    recomputed in sub-segments (a first pass stores sub-checkpoints, then each
+    sub-segment is taped) with identical results; a limit below every schedule
+    is refused before anything runs"""
+    dims = (16, 12, 20)
+    n = dims[0] * dims[1] * dims[2]
+    per_tape = 80 * n           # 10 doubles per node
+    res, plans = [], []
+    for extra in (None, 1280 * n, 1120 * n):
+        g, target = _small_lattice(cpmp, dims)
+        if extra is None:
+            plan = g.adjoint_plan(40, 40)
+            base = plan["device_bytes"] - 40 * per_tape   # everything but the tape
+            limit = 0
+        else:
+            limit = base + extra
+            plan = g.adjoint_plan(40, 40, memory_limit_bytes=limit)
+        plans.append(plan)
+        res.append(g.adjoint(40, target, checkpoint_every=40, memory_limit_bytes=limit))
+        assert g.step_count == 40
+        g.close()
+    print(f"[adjoint-plan] {plans}", flush=True)
+    # total(S) = base + S * per_tape + max(ceil(40 / S) - 2, 0) * (160 B per node)
+    assert (plans[0]["sub_length"], plans[0]["sub_checkpoints"], plans[0]["recomputed_steps"]) == (40, 0, 40)
+    assert (plans[1]["sub_length"], plans[1]["sub_checkpoints"], plans[1]["recomputed_steps"]) == (14, 1, 68)
+    assert (plans[2]["sub_length"], plans[2]["sub_checkpoints"], plans[2]["recomputed_steps"]) == (10, 2, 70)
+    assert plans[1]["device_bytes"] == base + 1280 * n and plans[2]["device_bytes"] == base + 1120 * n
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        assert np.array_equal(r[1], res[0][1])
+        assert np.array_equal(r[2], res[0][2])
+        assert r[3] == res[0][3]
+    g, target = _small_lattice(cpmp, dims)
+    with pytest.raises(cpmp.CPMPError):
+        g.adjoint_plan(40, 40, memory_limit_bytes=base + 400 * n)
+    with pytest.raises(cpmp.CPMPError):
+        g.adjoint(40, target, checkpoint_every=40, memory_limit_bytes=base + 400 * n)
+    assert g.step_count == 0
+    g.close()
+
+
+@pytest.mark.parametrize("dims", [(16, 12, 20), (32, 16, 32)])
+def test_adjoint_fp32_form(cpmp, dims):
+    """FP32 form (32 x 16 x 32 takes the fused-density path): the reverse sweep
+    runs in FP64 along the FP32 trajectory.  J and the gradients follow the FP64
+    form's to the FP32 rounding of the trajectory, and every schedule (checkpoint
+    interval, tape budget, sub-segments) gives identical results."""
+    n = dims[0] * dims[1] * dims[2]
+    g, target = _small_lattice(cpmp, dims, "fp64")
+    ref = g.adjoint(40, target, checkpoint_every=10)
+    g.close()
+    res = []
+    per_tape = 40 * n           # 10 FP32 values per node
+    for ck, budget, sub in ((10, 0, False), (40, 0, False), (7, 13 * per_tape, False), (40, 0, True)):
+        g, target = _small_lattice(cpmp, dims, "fp32")
+        limit = 0
+        if sub:
+            plan = g.adjoint_plan(40, 40)
+            assert plan["sub_length"] == 40
+            base = plan["device_bytes"] - 40 * per_tape
+            limit = base + 16 * per_tape
+            assert g.adjoint_plan(40, 40, memory_limit_bytes=limit)["sub_length"] < 40
+        res.append(g.adjoint(40, target, checkpoint_every=ck, tape_budget_bytes=budget, memory_limit_bytes=limit))
+        assert g.step_count == 40
+        g.close()
+    J, glo, ghi, gka = res[0]
+    eg = np.max(np.abs(glo - ref[1])) / np.max(np.abs(ref[1]))
+    print(f"[adjoint-fp32] {dims}: J {J:.10e} vs FP64 {ref[0]:.10e}; gradient rel {eg:.2e}; "
+          f"dJ/dKa {gka:.6e} vs {ref[3]:.6e}", flush=True)
+    assert J == pytest.approx(ref[0], rel=1e-5)
+    assert eg < 1e-3
+    assert np.max(np.abs(ghi - ref[2])) <= 1e-3 * np.max(np.abs(ref[2]))
+    assert gka == pytest.approx(ref[3], rel=1e-3)
+    for r in res[1:]:
+        assert r[0] == J and np.array_equal(r[1], glo) and np.array_equal(r[2], ghi) and r[3] == gka

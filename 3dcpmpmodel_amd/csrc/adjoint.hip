@@ -25,6 +25,8 @@
 #include "device_common.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -150,6 +152,23 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
     Wf[I] = wx; Wf[N + I] = wy; Wf[2 * N + I] = wz;
 }
 
+// sum_a G_a(A_a) for the isotropic gradient G (axis and face-diagonal 6-point
+// differences dd, weights 2:1:1:1:1, /360).  Each face diagonal appears in two
+// components; since dd(a, -u) = -dd(a, u) the pair is one difference of a sum or
+// difference of two components, e.g. dd(Ax, (1,-1,0)) + dd(Ay, (-1,1,0)) =
+// dd(Ax - Ay, (1,-1,0)): nine differences instead of fifteen.
+__device__ __forceinline__ double grad_divergence(const Stencil<double, double> &st, const double *ax,
+                                                  const double *ay, const double *az) {
+    auto dd2 = [&](const double *a, const double *b, double sg, int ux, int uy, int uz) {
+        auto v = [&](int s) { return st.at(a, s * ux, s * uy, s * uz) + sg * st.at(b, s * ux, s * uy, s * uz); };
+        return -v(-3) + 9 * v(-2) - 45 * v(-1) + 45 * v(1) - 9 * v(2) + v(3);
+    };
+    const double axis = st.dd(ax, 1, 0, 0) + st.dd(ay, 0, 1, 0) + st.dd(az, 0, 0, 1);
+    const double diag = dd2(ax, ay, 1, 1, 1, 0) + dd2(ax, ay, -1, 1, -1, 0) + dd2(ax, az, 1, 1, 0, 1)
+                        + dd2(ax, az, -1, 1, 0, -1) + dd2(ay, az, 1, 0, 1, 1) + dd2(ay, az, -1, 0, 1, -1);
+    return (2 * axis + diag) * (1.0 / 360.0);
+}
+
 // transposed force stencils, all nodes: mub = sum_a G_a(P_a), rb_new = rb_loc - sum_a G_a(W_a)
 __global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const double *__restrict__ Pf,
                                                      const double *__restrict__ Wf, const double *__restrict__ rb_loc,
@@ -162,8 +181,105 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const do
     int xs[7], ys[7], zs[7];
     wrap7(P, c, xs, ys, zs);
     const Stencil<double, double> st{P, xs, ys, zs};
-    mub[I] = st.gx(Pf) + st.gy(Pf + N) + st.gz(Pf + 2 * N);
-    rb_new[I] = rb_loc[I] - (st.gx(Wf) + st.gy(Wf + N) + st.gz(Wf + 2 * N));
+    mub[I] = grad_divergence(st, Pf, Pf + N, Pf + 2 * N);
+    rb_new[I] = rb_loc[I] - grad_divergence(st, Wf, Wf + N, Wf + 2 * N);
+}
+
+// The same transposed stencils as a march along x, one field triple per launch
+// (Pf -> mub, Wf -> rb_new).  A workgroup owns a tile of kTileY x kTileZ columns
+// and a chunk [xa, xb) of planes; it streams the planes xa-3 .. xb+2 of the
+// triple (with a halo of 3 in y and z) through LDS one at a time, and each thread
+// adds the contributions of the plane in LDS to the seven output planes that
+// reach it (x' - 3 .. x' + 3), kept as running sums in registers; output plane
+// x' - 3 is then complete and written.  Every value is read from memory once per
+// chunk instead of once per stencil point.  With the coefficients c_s of the
+// 6-point difference (c_1 = 45, c_2 = -9, c_3 = 1, c_-s = -c_s) the
+// contribution of source plane x' = x + s to output x is
+//   c_s [2 Ax(y,z) + (Ax+Ay)(y+s,z) + (Ax-Ay)(y-s,z) + (Ax+Az)(y,z+s) + (Ax-Az)(y,z-s)]
+// and the in-plane directions add to output x' itself
+//   sum_s c_s [2 Ay(y+s,z) + 2 Az(y,z+s) + (Ay+Az)(y+s,z+s) + (Ay-Az)(y+s,z-s)],
+// the sum of grad_divergence above (summed in another order).
+// SUB: out = base - sum / 360 (rb_new from rb_loc); otherwise out = sum / 360.
+constexpr int kAdjHalo = 3;
+constexpr int kAdjEY = kTileY + 2 * kAdjHalo, kAdjEZ = kTileZ + 2 * kAdjHalo, kAdjEA = kAdjEY * kAdjEZ;
+constexpr int kAdjSlots = (kAdjEA + TPB - 1) / TPB;
+static_assert(kTileY * kTileZ == TPB, "one thread per tile column");
+
+template <bool SUB>
+__global__ void __launch_bounds__(TPB) k_adj_stencil_march(const DevParams P, const double *__restrict__ A,
+                                                           const double *__restrict__ base, double *__restrict__ out,
+                                                           int ntz, int nty, int xc) {
+    __shared__ double sf[3][kAdjEA];   // x, y, z components of one plane
+    const long long N = P.N, plane = (long long)P.ny * P.nz;
+    const int tid = threadIdx.x, tz = tid % kTileZ, ty = tid / kTileZ;
+    int b = blockIdx.x;
+    const int zt = b % ntz;
+    b /= ntz;
+    const int yt = b % nty, ch = b / nty;
+    const int z0 = zt * kTileZ, y0 = yt * kTileY, xa = ch * xc, xb = min(xa + xc, P.nx);
+    const int y = y0 + ty, z = z0 + tz;
+    const bool own = y < P.ny && z < P.nz;
+    // LDS slots this thread fills: position i = tid + TPB r of the haloed tile
+    long long off[kAdjSlots];
+    int pos[kAdjSlots];
+#pragma unroll
+    for (int r = 0; r < kAdjSlots; ++r) {
+        const int i = tid + TPB * r;
+        pos[r] = i < kAdjEA ? i : -1;
+        const int ii = i < kAdjEA ? i : 0;
+        off[r] = (long long)wrapc(y0 - kAdjHalo + ii / kAdjEZ, P.ny) * P.nz + wrapc(z0 - kAdjHalo + ii % kAdjEZ, P.nz);
+    }
+    double pre[kAdjSlots][3];
+    auto fetch = [&](int xp) {
+        const double *src = A + (long long)wrapc(xp, P.nx) * plane;
+#pragma unroll
+        for (int r = 0; r < kAdjSlots; ++r)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pre[r][a] = src[a * N + off[r]];
+    };
+    double acc[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) acc[k] = 0;
+    const int c0 = (ty + kAdjHalo) * kAdjEZ + tz + kAdjHalo;
+    const double *ax = sf[0], *ay = sf[1], *az = sf[2];
+    fetch(xa - kAdjHalo);
+    for (int xp = xa - kAdjHalo; xp <= xb + kAdjHalo - 1; ++xp) {
+        __syncthreads();   // the previous plane is no longer read
+#pragma unroll
+        for (int r = 0; r < kAdjSlots; ++r)
+            if (pos[r] >= 0)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) sf[f][pos[r]] = pre[r][f];
+        __syncthreads();
+        fetch(xp + 1);     // next plane in flight during the arithmetic (wraps past the end)
+        const double x0 = 2 * ax[c0];
+        double inpl = 0;
+#pragma unroll
+        for (int s = 1; s <= 3; ++s) {
+            const double cs = s == 1 ? 45.0 : (s == 2 ? -9.0 : 1.0);
+            const int dy = s * kAdjEZ;
+            const double axp = ax[c0 + dy], axm = ax[c0 - dy], ayp = ay[c0 + dy], aym = ay[c0 - dy];
+            const double axr = ax[c0 + s], axl = ax[c0 - s], azr = az[c0 + s], azl = az[c0 - s];
+            // source plane x' = x + s (output x = x' - s, slot 3 - s) and x' = x - s (slot 3 + s)
+            const double vp = x0 + (axp + ayp) + (axm - aym) + (axr + azr) + (axl - azl);
+            const double vm = x0 + (axm + aym) + (axp - ayp) + (axl + azl) + (axr - azr);
+            acc[3 - s] += cs * vp;
+            acc[3 + s] -= cs * vm;
+            // in-plane directions, output x'
+            const double dpp = ay[c0 + dy + s] + az[c0 + dy + s], dmm = ay[c0 - dy - s] + az[c0 - dy - s];
+            const double dpm = ay[c0 + dy - s] - az[c0 + dy - s], dmp = ay[c0 - dy + s] - az[c0 - dy + s];
+            inpl += cs * (2 * (ayp - aym) + 2 * (azr - azl) + (dpp - dmm) + (dpm - dmp));
+        }
+        acc[3] += inpl;
+        if (xp >= xa + kAdjHalo && own) {   // output plane xp - 3 is complete
+            const long long I = (long long)(xp - kAdjHalo) * plane + (long long)y * P.nz + z;
+            const double v = acc[0] * (1.0 / 360.0);
+            out[I] = SUB ? base[I] - v : v;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] = acc[k + 1];
+        acc[6] = 0;
+    }
 }
 
 // transposed chemical potential: Laplacian of the fluid mu cotangent, EOS slope,
@@ -440,6 +556,17 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const DevParams &P = c->dp;
     const dim3 gwt((P.nx + kWallTile - 1) / kWallTile, (P.ny + kWallTile - 1) / kWallTile, 2), bwt(kWallTile * kWallTile);
     const int passes = std::max(P.lo, P.hi);
+    // transposed force stencils: x march over tiles and chunks of planes (at least
+    // 1024 workgroups), when the chunks are at least 8 planes long (the 6 halo
+    // planes per chunk cost less than the cache misses of the one-thread-per-node
+    // kernel); CPMP_ADJ_STENCIL=v1 / march forces either
+    const int ntz_a = (P.nz + kTileZ - 1) / kTileZ, nty_a = (P.ny + kTileY - 1) / kTileY;
+    int xc_a = 2;
+    for (int cand = 32; cand >= 2; cand /= 2)
+        if ((long long)((P.nx + cand - 1) / cand) * ntz_a * nty_a >= 1024) { xc_a = cand; break; }
+    const char *sv = std::getenv("CPMP_ADJ_STENCIL");
+    const bool march = sv && std::strcmp(sv, "v1") == 0 ? false : (sv && std::strcmp(sv, "march") == 0) || xc_a >= 8;
+    const unsigned march_blocks = (unsigned)((P.nx + xc_a - 1) / xc_a * ntz_a * nty_a);
     const int64_t step0 = c->step;
     const size_t fbytes = 19 * (size_t)N * c->pop_bytes, rbytes = (size_t)N * c->scal_bytes;
     // tape entry of step n, in the storage scalar type (FP64 or FP32): rho^n after
@@ -501,8 +628,14 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
                     using TT = std::remove_const_t<std::remove_pointer_t<decltype(tt)>>;
                     hipLaunchKernelGGL(k_adj_collide<TT>, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA,
                                        (const double *)rb, tt, tt + N, tt + 4 * N, fbB, rbl, Pf, Wf);
-                    hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf,
-                                       (const double *)Wf, (const double *)rbl, mub, rbn);
+                    if (march) {
+                        hipLaunchKernelGGL(k_adj_stencil_march<false>, dim3(march_blocks), dim3(TPB), 0, s, P,
+                                           (const double *)Pf, (const double *)nullptr, mub, ntz_a, nty_a, xc_a);
+                        hipLaunchKernelGGL(k_adj_stencil_march<true>, dim3(march_blocks), dim3(TPB), 0, s, P,
+                                           (const double *)Wf, (const double *)rbl, rbn, ntz_a, nty_a, xc_a);
+                    } else
+                        hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf,
+                                           (const double *)Wf, (const double *)rbl, mub, rbn);
                     hipLaunchKernelGGL(k_adj_mu<TT>, node_grid(P), dim3(TPB), 0, s, P, tt, (const double *)mub, rbn, kpart);
                 };
                 if (sb == sizeof(double)) tape_kernels((const double *)t0);

@@ -370,27 +370,31 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
         const int ly = e / MLZ, lz = e - ly * MLZ;
         hoff[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) * P.nz + wrapc(z0 - MH + lz, P.nz) : 0;
     }
-    TS hr[HE];
-    auto fetch_plane = [&](int xg) {
+    // two planes in flight: the halo of plane x+4 and x+5 while plane x computes
+    // (register sets A and B alternate; the loop is unrolled by two so that they
+    // keep fixed registers)
+    TS hA[HE], hB[HE];
+    auto fetch_plane = [&](int xg, TS *hr) {   // wraps past the chunk (loads unused then)
         const long long pb = (long long)wrapc(xg, P.nx) * P.ny * P.nz;
 #pragma unroll
         for (int h = 0; h < HE; ++h)
             if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) hr[h] = rho[pb + hoff[h]];
     };
-    auto store_plane = [&](int slot) {
+    auto store_plane = [&](int slot, const TS *hr) {
 #pragma unroll
         for (int h = 0; h < HE; ++h)
             if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) s_rho[slot][threadIdx.x + h * TPB] = hr[h];
     };
-    for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k); store_plane(k + MH); }
-    fetch_plane(xa + MH);
+    for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k, hA); store_plane(k + MH, hA); }
+    fetch_plane(xa + MH, hA);
+    fetch_plane(xa + MH + 1, hB);
     const int base = (ty + MH) * MLZ + tz + MH;
     const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
-    for (int x = xa; x < xb; ++x) {
+    auto plane_step = [&](int x, TS *hr) {
         __syncthreads();
-        store_plane((x - xa + 2 * MH) % 7);
+        store_plane((x - xa + 2 * MH) % 7, hr);   // plane x+3
         __syncthreads();
-        if (x + 1 < xb) fetch_plane(x + 1 + MH);
+        fetch_plane(x + 2 + MH, hr);              // plane x+5
         const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
         if (active) {
             int slot[7];
@@ -400,7 +404,13 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
         } else if (rho_solid && inr) {
             rho_solid[I] = s_rho[(x - xa + 3) % 7][base];
         }
+    };
+    int x = xa;
+    for (; x + 1 < xb; x += 2) {
+        plane_step(x, hA);
+        plane_step(x + 1, hB);
     }
+    if (x < xb) plane_step(x, hA);
 }
 
 // With RS the kernel also sums the next step's density: every post-collision

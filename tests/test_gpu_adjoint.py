@@ -317,3 +317,27 @@ def test_adjoint_fp32_form(cpmp, dims):
     assert gka == pytest.approx(ref[3], rel=1e-3)
     for r in res[1:]:
         assert r[0] == J and np.array_equal(r[1], glo) and np.array_equal(r[2], ghi) and r[3] == gka
+
+
+@pytest.mark.parametrize("dims", [(16, 12, 20), (15, 17, 40)])
+def test_adjoint_stencil_march_matches_per_node(cpmp, dims):
+    """the transposed force stencils as an x march (partial y / z tiles, chunks
+    of two planes, a last chunk of one plane for nx = 15) against the
+    one-thread-per-node kernel: the same sums in another order"""
+    import os
+    res = {}
+    for mode in ("v1", "march"):
+        os.environ["CPMP_ADJ_STENCIL"] = mode
+        try:
+            g, target = _small_lattice(cpmp, dims)
+            res[mode] = g.adjoint(30, target, checkpoint_every=10)
+            g.close()
+        finally:
+            os.environ.pop("CPMP_ADJ_STENCIL", None)
+    a, b = res["v1"], res["march"]
+    assert a[0] == b[0]                       # the forward run is the same
+    for k in (1, 2):
+        err = np.max(np.abs(a[k] - b[k])) / np.max(np.abs(a[k]))
+        print(f"[adjoint-march] {dims} gradient {k}: rel {err:.2e}", flush=True)
+        assert err < 1e-11
+    assert b[3] == pytest.approx(a[3], rel=1e-11)

@@ -296,8 +296,12 @@ def main():
     ap.add_argument("--forms", default="fp32,fp64,fp16")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--sessile-steps", type=int, default=4000)
+    ap.add_argument("--adjoint", action="store_true", help="run the adjoint cost leg also with --no-accuracy")
     ap.add_argument("--large", action="store_true",
                     help="also run the 512x512x256 FP32 wetting run with adjoint (about 2 minutes, ~270 GB)")
+    ap.add_argument("--large-fwd", type=int, default=10000, help="forward steps of the large run")
+    ap.add_argument("--large-adj", type=int, default=2000, help="steps the large run's adjoint covers")
+    ap.add_argument("--large-every", type=int, default=100, help="checkpoint interval of the large run's adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -326,13 +330,14 @@ def main():
         log("[bench] accuracy of the FP32 / FP16 forms against FP64 ...")
         acc = accuracy(cpmp, args.sessile_steps)
         log(f"[bench] contact angle err {acc['contact_angle_err_deg']}, coexistence {acc['coexistence_rel_err']}")
+    if not args.no_accuracy or args.adjoint:
         log("[bench] adjoint cost ...")
         adj = adjoint_cost(cpmp)
         log(f"[bench] adjoint/forward = {adj['adjoint_over_forward']:.2f} (checkpoints + recompute), "
             f"{adj['adjoint_taped_over_forward']:.2f} (tape in device memory)")
     large = None
     if args.large:
-        large = large_run(cpmp)
+        large = large_run(cpmp, args.large_fwd, args.large_adj, args.large_every)
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")

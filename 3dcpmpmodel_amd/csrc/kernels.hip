@@ -334,11 +334,15 @@ struct LdsStencil {
         return -at(-3 * ux, -3 * uy, -3 * uz) + TC(9) * at(-2 * ux, -2 * uy, -2 * uz) - TC(45) * at(-ux, -uy, -uz)
                + TC(45) * at(ux, uy, uz) - TC(9) * at(2 * ux, 2 * uy, 2 * uz) + at(3 * ux, 3 * uy, 3 * uz);
     }
+    // nine directional differences: each face diagonal serves two components
+    // (dd(-u) = -dd(u) for the antisymmetric 6-point difference)
     __device__ __forceinline__ void grad(TC &gx, TC &gy, TC &gz) const {
         const TC s = TC(1.0 / 360.0);
-        gx = (TC(2) * dd(1, 0, 0) + dd(1, 0, 1) + dd(1, 0, -1) + dd(1, 1, 0) + dd(1, -1, 0)) * s;
-        gy = (TC(2) * dd(0, 1, 0) + dd(0, 1, 1) + dd(0, 1, -1) + dd(1, 1, 0) + dd(-1, 1, 0)) * s;
-        gz = (TC(2) * dd(0, 0, 1) + dd(1, 0, 1) + dd(-1, 0, 1) + dd(0, 1, 1) + dd(0, -1, 1)) * s;
+        const TC xy = dd(1, 1, 0), xmy = dd(1, -1, 0), xz = dd(1, 0, 1), xmz = dd(1, 0, -1);
+        const TC yz = dd(0, 1, 1), ymz = dd(0, 1, -1);
+        gx = (TC(2) * dd(1, 0, 0) + xz + xmz + xy + xmy) * s;
+        gy = (TC(2) * dd(0, 1, 0) + yz + ymz + xy - xmy) * s;
+        gz = (TC(2) * dd(0, 0, 1) + xz - xmz + yz - ymz) * s;
     }
 };
 
@@ -544,16 +548,17 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
         }
     };
     // arrivals at a ring position (ly, lz) of the tile border (accumulator frame)
+    // (branch-free: sources outside the tile read slot 0 and contribute 0)
     auto collect = [&](int ly, int lz, TC &pm, TC &pc, TC &pn) {
 #pragma unroll
         for (int q = 3; q < Q; ++q) {
             const int sy = ly - 1 - cy(q), sz = lz - 1 - cz(q);
-            if (sy >= 0 && sy < MTY && sz >= 0 && sz < MTZ) {
-                const TC v = s_mail[q - 3][sy * MTZ + sz];
-                if (cx(q) == 0) pc += v;
-                else if (cx(q) > 0) pn += v;
-                else pm += v;
-            }
+            const bool in = sy >= 0 && sy < MTY && sz >= 0 && sz < MTZ;
+            const TC v = s_mail[q - 3][in ? sy * MTZ + sz : 0];
+            const TC w = in ? v : TC(0);
+            if (cx(q) == 0) pc += w;
+            else if (cx(q) > 0) pn += w;
+            else pm += w;
         }
     };
     // plane xp (xa-1 <= xp <= xb) is complete over this tile and chunk: write it out
@@ -629,15 +634,15 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
                     TC w;
                     w = pop_load<TF, TC>(&sv, 0, q, tref);
                     const int qx = cx(q), qy = cy(q), qz = cz(q);
-                    if (qy == 0 && qz == 0) {
+                    if (qy == 0 && qz == 0) {           // resolved per q at compile time
                         if (qx == 0) acc_cur += w;
                         else if (qx > 0) acc_next += w;
                         else acc_prev += w;
-                    } else if ((qz < 0 && bb_m) || (qz > 0 && bb_p)) {
-                        acc_cur += w;                   // bounced back: pulled by this node as opp(q)
-                        s_mail[q - 3][threadIdx.x] = TC(0);
                     } else {
-                        s_mail[q - 3][threadIdx.x] = w;
+                        // bounced back: pulled by this node as opp(q) (selects, no branch)
+                        const bool bnc = (qz < 0 && bb_m) || (qz > 0 && bb_p);
+                        acc_cur += bnc ? w : TC(0);
+                        s_mail[q - 3][threadIdx.x] = bnc ? TC(0) : w;
                     }
                 }
             }

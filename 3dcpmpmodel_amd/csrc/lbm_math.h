@@ -164,11 +164,21 @@ CPMP_HD void momentum(const T f[Q], T &jx, T &jy, T &jz) {
 // j = (m3, m5, m7) and rho the stored density; it is returned in u[3].  Each row
 // is finished as soon as its equilibrium values are formed, which keeps the
 // number of live registers low.
+// 1/x: in FP32 device code the hardware reciprocal (v_rcp_f32, 1 ulp), the
+// correctly rounded quotient elsewhere (FP64, host)
+template <class T>
+CPMP_HD T recip(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+#endif
+    return T(1) / x;
+}
+
 template <class T>
 CPMP_HD void collide(const T f[Q], T rho, T Fx, T Fy, T Fz, const T *ts, const T *qp, T out[Q], T u[3]) {
     T m[Q];
     moments(f, m);
-    const T inv = T(1) / rho;
+    const T inv = recip(rho);
     const T ux = m[3] * inv, uy = m[5] * inv, uz = m[7] * inv;
     u[0] = ux; u[1] = uy; u[2] = uz;
     const T ex = ux + Fx * inv, ey = uy + Fy * inv, ez = uz + Fz * inv;

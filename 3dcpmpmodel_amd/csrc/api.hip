@@ -63,6 +63,9 @@ void fill_devparams(cpmp_ctx *c) {
     d.qp_on = (p.K1 != 0.0 || p.K2 != 0.0) ? 1 : 0;
     d.A = p.A; d.B = p.B; d.T = p.T; d.K2s = p.K * p.K; d.Ka = p.Ka; d.K1 = p.K1; d.K2 = p.K2;
     for (int q = 0; q < 19; ++q) { d.ts[q] = p.Ts[q]; d.tsf[q] = (float)p.Ts[q]; }
+    d.unit_rates = 1;
+    for (int q = 0; q < 19; ++q)
+        if (cpmp::unit_rate_row(q) && p.Ts[q] != 1.0) d.unit_rates = 0;
     d.rho0 = p.rho0;
     d.xc = march_chunk(p.nx, p.ny, p.nz);
 }

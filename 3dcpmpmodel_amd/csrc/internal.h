@@ -25,6 +25,7 @@ struct DevParams {
     double rho0;                // FP16 shift density
     int tiled, nty, ntz;        // tile-contiguous population sites (pop_site), tile counts in y, z
     int xc;                     // x planes per workgroup of the march kernels
+    int unit_rates;             // the rates of rows 0, 1, 3, 5, 7, 10, 12, 16-18 are all 1 (reference set)
 };
 
 enum Kern { KW = CPMP_K_WALL, KM = CPMP_K_MU, KC = CPMP_K_COLLIDE, KR = CPMP_K_MACRO };

@@ -443,7 +443,7 @@ __device__ __forceinline__ int ring_index(int ly, int lz) {
     return 2 * AZ + 2 * (ly - 1) + (lz == AZ - 1 ? 1 : 0);
 }
 
-template <class TF, class TS, class TC, bool MV, bool RS>
+template <class TF, class TS, class TC, bool MV, bool RS, bool UR>
 __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
@@ -598,9 +598,9 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
             if (P.qp_on) {
                 TC qp[Q];
                 qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
-                collide(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
+                collide<TC, UR>(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
             } else {
-                collide(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
+                collide<TC, UR>(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
             }
             if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
                 const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
@@ -1205,18 +1205,24 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v) {
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + c->dp.xc - 1) / c->dp.xc;
             const dim3 grid((unsigned)(ntz * nty * nch));
             constexpr bool can_rs = !std::is_same<TF, double>::value;   // FP64 keeps the separate pass
-            if (can_rs && rs && want_mv)
-                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
-            else if (can_rs && rs)
-                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, can_rs>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
-            else if (want_mv)
-                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, true, false>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
-            else
-                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, false, false>), grid, dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb, cb, remap, ref_buffer<TS>(c));
+            // MV, RS and the unit-rate specialisation as template arguments
+            auto go = [&](auto mv_t, auto rs_t, auto ur_t) {
+                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, decltype(mv_t)::value, decltype(rs_t)::value,
+                                                    decltype(ur_t)::value>),
+                                   grid, dim3(TPB), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb,
+                                   cb, remap, ref_buffer<TS>(c));
+            };
+            using T1 = std::true_type;
+            using F0 = std::false_type;
+            using RSB = std::integral_constant<bool, can_rs>;
+            const bool ur = c->dp.unit_rates != 0;
+            if (can_rs && rs) {
+                if (want_mv) { if (ur) go(T1{}, RSB{}, T1{}); else go(T1{}, RSB{}, F0{}); }
+                else { if (ur) go(F0{}, RSB{}, T1{}); else go(F0{}, RSB{}, F0{}); }
+            } else {
+                if (want_mv) { if (ur) go(T1{}, F0{}, T1{}); else go(T1{}, F0{}, F0{}); }
+                else { if (ur) go(F0{}, F0{}, T1{}); else go(F0{}, F0{}, F0{}); }
+            }
         }
         return hipGetLastError();
     });

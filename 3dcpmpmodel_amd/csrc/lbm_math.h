@@ -89,8 +89,9 @@ CPMP_HD void moments(const T f[Q], T m[Q]) {
     m[18] = bxz - byz;
 }
 
-// f = R m = M^T diag(1/n2) m, given a[r] = m[r] / n2[r].
-template <class T>
+// f = R m = M^T diag(1/n2) m, given a[r] = m[r] / n2[r].  Z16: a[16..18] are zero
+// and their terms are left out.
+template <class T, bool Z16 = false>
 CPMP_HD void inverse_scaled(const T a[Q], T f[Q]) {
     f[0] = a[0] - T(30) * a[1] + T(12) * a[2];
     const T Ca = a[0] - T(11) * a[1] - T(4) * a[2];
@@ -106,6 +107,21 @@ CPMP_HD void inverse_scaled(const T a[Q], T f[Q]) {
     const T Exy = Cd + a[9] + a[10] + a[11] + a[12];
     const T Exz = Cd + a[9] + a[10] - a[11] - a[12];
     const T Eyz = Cd - T(2) * (a[9] + a[10]);
+    if constexpr (Z16) {
+        f[7] = Exy + X + Y + a[13];
+        f[8] = Exy - X + Y - a[13];
+        f[9] = Exy + X - Y - a[13];
+        f[10] = Exy - X - Y + a[13];
+        f[11] = Exz + X + Z + a[15];
+        f[12] = Exz - X + Z - a[15];
+        f[13] = Exz + X - Z - a[15];
+        f[14] = Exz - X - Z + a[15];
+        f[15] = Eyz + Y + Z + a[14];
+        f[16] = Eyz - Y + Z - a[14];
+        f[17] = Eyz + Y - Z - a[14];
+        f[18] = Eyz - Y - Z + a[14];
+        return;
+    }
     // xy: f7 (+,+) f8 (-,+) f9 (+,-) f10 (-,-)
     f[7] = Exy + X + Y + a[13] + a[16] - a[17];
     f[8] = Exy - X + Y - a[13] - a[16] - a[17];
@@ -121,6 +137,13 @@ CPMP_HD void inverse_scaled(const T a[Q], T f[Q]) {
     f[16] = Eyz - Y + Z - a[14] - a[17] - a[18];
     f[17] = Eyz + Y - Z - a[14] + a[17] + a[18];
     f[18] = Eyz - Y - Z + a[14] - a[17] + a[18];
+}
+
+// Rows whose relaxation rate is 1 in the reference set (FluidCuda.cu:154-157):
+// 0, 1, 3, 5, 7, 10, 12, 16, 17, 18.  With rate 1 a row's post-collision moment is
+// its forced equilibrium alone (and 0 for rows 16-18).
+CPMP_HD constexpr bool unit_rate_row(int r) {
+    return r == 0 || r == 1 || r == 3 || r == 5 || r == 7 || r == 10 || r == 12 || r >= 16;
 }
 
 // Equilibrium moments Meq(rho,u) of FluidCuda.cu:268-286 (rows 16-18 are 0).
@@ -174,7 +197,9 @@ CPMP_HD T recip(T x) {
     return T(1) / x;
 }
 
-template <class T>
+// UR: the rates of the unit_rate_row rows are 1 (the caller has checked); those rows
+// skip (I - S)(m - Meq), which is exactly 0, and their moments are not formed.
+template <class T, bool UR = false>
 CPMP_HD void collide(const T f[Q], T rho, T Fx, T Fy, T Fz, const T *ts, const T *qp, T out[Q], T u[3]) {
     T m[Q];
     moments(f, m);
@@ -187,7 +212,7 @@ CPMP_HD void collide(const T f[Q], T rho, T Fx, T Fy, T Fz, const T *ts, const T
     const T u11 = uy * uy - uz * uz, e11 = ey * ey - ez * ez;
     T a[Q];
     auto row = [&](int r, T mequ, T meqe) {
-        T v = (T(1) - ts[r]) * (m[r] - mequ) + meqe;
+        T v = (UR && unit_rate_row(r)) ? meqe : (T(1) - ts[r]) * (m[r] - mequ) + meqe;
         if (qp) v += ts[r] * qp[r];
         a[r] = v * T(1.0 / n2(r));
     };
@@ -209,8 +234,8 @@ CPMP_HD void collide(const T f[Q], T rho, T Fx, T Fy, T Fz, const T *ts, const T
     row(14, rho * uy * uz, rho * ey * ez);
     row(15, rho * ux * uz, rho * ex * ez);
 #pragma unroll
-    for (int r = 16; r < Q; ++r) a[r] = (T(1) - ts[r]) * m[r] * T(1.0 / n2(r));
-    inverse_scaled(a, out);
+    for (int r = 16; r < Q; ++r) a[r] = UR ? T(0) : (T(1) - ts[r]) * m[r] * T(1.0 / n2(r));
+    inverse_scaled<T, UR>(a, out);
 }
 
 // Qp of FluidCuda.cu:260-266 from the density gradient (K1, K2 tuning).

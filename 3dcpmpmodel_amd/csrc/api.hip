@@ -394,10 +394,7 @@ int cpmp_step(cpmp_ctx *c, int nsteps) {
     if (nsteps < 0) return set_error(CPMP_EINVAL, "nsteps %d", nsteps);
     if (nsteps == 0) return CPMP_OK;
     HIPCHK(hipSetDevice(c->p.device));
-    for (int n = 0; n < nsteps; ++n) {
-        HIPCHK(launch_step(c, n == nsteps - 1));
-        ++c->step;
-    }
+    HIPCHK(launch_steps(c, nsteps));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->mv_valid = true;
     return CPMP_OK;
@@ -594,10 +591,7 @@ int cpmp_time_steps(cpmp_ctx *c, int nsteps, double *ms) {
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipEventRecord(a, c->stream));
-    for (int n = 0; n < nsteps; ++n) {
-        HIPCHK(launch_step(c, n == nsteps - 1));
-        ++c->step;
-    }
+    HIPCHK(launch_steps(c, nsteps));
     HIPCHK(hipEventRecord(b, c->stream));
     HIPCHK(hipEventSynchronize(b));
     float t = 0;

@@ -79,9 +79,12 @@ constexpr int kWallTile = 16;             // columns per edge of the wall-fill t
 hipError_t launch_pop_relayout(cpmp_ctx *c, const void *src, void *dst, bool to_device_layout);
 // uout (when given): u, grad mu, grad rho of every node, 9 N values of the
 // storage scalar type (adjoint tape)
-hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout = nullptr);
+// keep_sums (fused density): leave the tile sums in rho_next for k_mu_march<…, FIX>
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout = nullptr, bool keep_sums = false);
 hipError_t launch_macro(cpmp_ctx *c);
 hipError_t launch_step(cpmp_ctx *c, bool want_mv);
+// nsteps whole steps (MV on the last), advancing c->step
+hipError_t launch_steps(cpmp_ctx *c, int nsteps);
 hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, double rg, double rl, double solid);
 hipError_t launch_init_splash(cpmp_ctx *c, double radius, double width, double film, double rg, double rl);
 hipError_t launch_init_spheres(cpmp_ctx *c, const cpmp_init_desc *d);

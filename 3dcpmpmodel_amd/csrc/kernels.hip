@@ -346,88 +346,6 @@ struct LdsStencil {
     }
 };
 
-// Chemical potential as a march over the same tiles: rho of planes x-3..x+3 with an
-// in-plane halo of 3 sits in an LDS ring, so the 13 (or 19) stencil values of every
-// node come from LDS and each rho value is fetched from memory about once per tile.
-// Arithmetic of mu_node, as k_mu.
-template <class TS>
-__global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
-                                                  int ntz, int nty, TS *__restrict__ rho_solid, int xc) {
-    __shared__ TS s_rho[7][MLP];
-    int t = blockIdx.x;
-    if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
-    const int tzi = t % ntz;
-    t /= ntz;
-    const int tyi = t % nty;
-    const int ch = t / nty;
-    const int y0 = tyi * MTY, z0 = tzi * MTZ;
-    const int xa = ch * xc, xb = min(xa + xc, P.nx);
-    const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
-    const int y = y0 + ty, z = z0 + tz;
-    const bool inr = (y < P.ny) && (z < P.nz);
-    const bool active = inr && !is_solid(P, z);
-    constexpr int HE = (MLP + TPB - 1) / TPB;
-    int hoff[HE];
-#pragma unroll
-    for (int h = 0; h < HE; ++h) {
-        const int e = threadIdx.x + h * TPB;
-        const int ly = e / MLZ, lz = e - ly * MLZ;
-        hoff[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) * P.nz + wrapc(z0 - MH + lz, P.nz) : 0;
-    }
-    // two planes in flight: the halo of plane x+4 and x+5 while plane x computes
-    // (register sets A and B alternate; the loop is unrolled by two so that they
-    // keep fixed registers)
-    TS hA[HE], hB[HE];
-    auto fetch_plane = [&](int xg, TS *hr) {   // wraps past the chunk (loads unused then)
-        const long long pb = (long long)wrapc(xg, P.nx) * P.ny * P.nz;
-#pragma unroll
-        for (int h = 0; h < HE; ++h)
-            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) hr[h] = rho[pb + hoff[h]];
-    };
-    auto store_plane = [&](int slot, const TS *hr) {
-#pragma unroll
-        for (int h = 0; h < HE; ++h)
-            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) s_rho[slot][threadIdx.x + h * TPB] = hr[h];
-    };
-    for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k, hA); store_plane(k + MH, hA); }
-    fetch_plane(xa + MH, hA);
-    fetch_plane(xa + MH + 1, hB);
-    const int base = (ty + MH) * MLZ + tz + MH;
-    const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
-    auto plane_step = [&](int x, TS *hr) {
-        __syncthreads();
-        store_plane((x - xa + 2 * MH) % 7, hr);   // plane x+3
-        __syncthreads();
-        fetch_plane(x + 2 + MH, hr);              // plane x+5
-        const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
-        if (active) {
-            int slot[7];
-#pragma unroll
-            for (int dd = 0; dd < 7; ++dd) slot[dd] = (x - xa + dd) % 7;
-            mu[I] = mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + 3]][base + dy * MLZ + dz]; });
-        } else if (rho_solid && inr) {
-            rho_solid[I] = s_rho[(x - xa + 3) % 7][base];
-        }
-    };
-    int x = xa;
-    for (; x + 1 < xb; x += 2) {
-        plane_step(x, hA);
-        plane_step(x + 1, hB);
-    }
-    if (x < xb) plane_step(x, hA);
-}
-
-// With RS the kernel also sums the next step's density: every post-collision
-// value (as stored) is added to the node that will pull it (MacroCalculate,
-// FluidCuda.cu:351-372, summed on the emitting side).  Populations along x stay
-// in the thread's column (registers); in-plane shifts go to an LDS accumulator
-// ring of planes x-1, x, x+1 with a one-node border.  A plane is complete once
-// the plane after it has collided: its tile nodes are stored to rho_next (the
-// sums over this tile and chunk), its border ring to rbuf, and the two partial
-// planes outside the chunk (xa-1, xb) to cbuf.  k_density_fix then adds the
-// rings of the neighbouring tiles and the partial planes of the neighbouring
-// chunks, in a fixed order, with no atomics to global memory.  Requires
-// ny % MTY == nz % MTZ == nx % P.xc == 0 (fuse_geometry).
 constexpr int AY = MTY + 2, AZ = MTZ + 2, AP = AY * AZ, ARING = AP - MTY * MTZ;
 
 // accumulator position (ly, lz) of ring entry r: rows ly = 0 and ly = AY-1, then
@@ -443,6 +361,166 @@ __device__ __forceinline__ int ring_index(int ly, int lz) {
     return 2 * AZ + 2 * (ly - 1) + (lz == AZ - 1 ? 1 : 0);
 }
 
+// Chemical potential as a march over the same tiles: rho of planes x-3..x+3 with an
+// in-plane halo of 3 sits in an LDS ring, so the 13 (or 19) stencil values of every
+// node come from LDS and each rho value is fetched from memory about once per tile.
+// Arithmetic of mu_node, as k_mu.
+// FIX (periodic FP32 lattices with the density summed by the collide kernel): rho
+// holds the previous collide's tile sums, which this kernel completes on the fly
+// with the terms k_density_fix adds (the ring entry and, on a chunk's first / last
+// plane, the partial-plane entry of the y-, z- and diagonal neighbour tile holding
+// the node, then the tile's own partial-plane entry: the same order, with +0 for
+// absent terms, so the same values) as each plane enters the ring; the completed
+// density of the workgroup's own tile and chunk is written to rho_out.  This
+// replaces the separate completion pass.
+template <class TS, bool FIX>
+__global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
+                                                  int ntz, int nty, TS *__restrict__ rho_solid, int xc,
+                                                  const TS *__restrict__ rbuf, const TS *__restrict__ cbuf,
+                                                  TS *__restrict__ rho_out, int nch) {
+    __shared__ TS s_rho[7][MLP];
+    int t = blockIdx.x;
+    if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
+    const int tzi = t % ntz;
+    t /= ntz;
+    const int tyi = t % nty;
+    const int ch = t / nty;
+    const int y0 = tyi * MTY, z0 = tzi * MTZ;
+    const int xa = ch * xc, xb = min(xa + xc, P.nx);
+    const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
+    const int y = y0 + ty, z = z0 + tz;
+    const bool inr = (y < P.ny) && (z < P.nz);
+    const bool active = inr && !is_solid(P, z);
+    constexpr int HE = (MLP + TPB - 1) / TPB;
+    int hoff[HE], hy[HE], hz[HE];
+    bool hown[HE];
+    // FIX: in-plane part of the ring index of the (up to three) neighbouring tiles
+    // holding each halo node (y-, z-, diagonal neighbour), -1 where absent
+    int ro[FIX ? HE : 1][3];
+    // and of their partial-plane entries (slot 3: the node's own tile), without the
+    // chunk and side, which are uniform per plane
+    int co[FIX ? HE : 1][4];
+    const long long plane_ring = (long long)nty * ntz * ARING;
+#pragma unroll
+    for (int h = 0; h < HE; ++h) {
+        const int e = threadIdx.x + h * TPB;
+        const int ly = e / MLZ, lz = e - ly * MLZ;
+        hy[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) : 0;
+        hz[h] = e < MLP ? wrapc(z0 - MH + lz, P.nz) : 0;
+        hoff[h] = hy[h] * P.nz + hz[h];
+        hown[h] = e < MLP && ly >= MH && ly < MH + MTY && lz >= MH && lz < MH + MTZ;
+        if constexpr (FIX) {
+            const int gyi = hy[h] / MTY, gzi = hz[h] / MTZ, gy = hy[h] % MTY, gz = hz[h] % MTZ;
+            const bool ey0 = gy == 0, ey1 = gy == MTY - 1, ez0 = gz == 0, ez1 = gz == MTZ - 1;
+            auto set = [&](int i, bool on, int dy, int dz, int ly2, int lz2) {
+                if (!on) return;
+                const int yy = (gyi + dy + nty) % nty, zz = (gzi + dz + ntz) % ntz;
+                ro[h][i] = (yy * ntz + zz) * ARING + ring_index(ly2, lz2);
+                co[h][i] = (yy * ntz + zz) * 2 * AP + ly2 * AZ + lz2;
+            };
+            ro[h][0] = ro[h][1] = ro[h][2] = -1;
+            co[h][0] = co[h][1] = co[h][2] = 0;
+            set(0, ey0, -1, 0, AY - 1, gz + 1);
+            set(0, ey1, 1, 0, 0, gz + 1);
+            set(1, ez0, 0, -1, gy + 1, AZ - 1);
+            set(1, ez1, 0, 1, gy + 1, 0);
+            set(2, ey0 && ez0, -1, -1, AY - 1, AZ - 1);
+            set(2, ey0 && ez1, -1, 1, AY - 1, 0);
+            set(2, ey1 && ez0, 1, -1, 0, AZ - 1);
+            set(2, ey1 && ez1, 1, 1, 0, 0);
+            co[h][3] = (gyi * ntz + gzi) * 2 * AP + (gy + 1) * AZ + gz + 1;
+        }
+    }
+    // two planes in flight: the halo of plane x+4 and x+5 while plane x computes
+    // (register sets A and B alternate; the loop is unrolled by two so that they
+    // keep fixed registers)
+    TS hA[HE], hB[HE];
+    TS fA[FIX ? HE : 1][3], fB[FIX ? HE : 1][3];
+    auto fetch_plane = [&](int xg, TS *hr, TS (*fv)[3]) {   // wraps past the chunk (loads unused then)
+        xg = wrapc(xg, P.nx);
+        const long long pb = (long long)xg * P.ny * P.nz;
+        const TS *rp = FIX ? rbuf + (long long)xg * plane_ring : nullptr;   // ring entries of plane xg
+#pragma unroll
+        for (int h = 0; h < HE; ++h)
+            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) {
+                hr[h] = rho[pb + hoff[h]];
+                if constexpr (FIX) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) fv[h][i] = ro[h][i] >= 0 ? rp[ro[h][i]] : TS(0);
+                }
+            }
+    };
+    auto store_plane = [&](int slot, int xg, const TS *hr, TS (*fv)[3]) {
+        const int xw = wrapc(xg, P.nx), k = xw % xc;
+        const bool chunk_end = k == 0 || k == xc - 1;   // partial-plane terms too (uniform)
+#pragma unroll
+        for (int h = 0; h < HE; ++h)
+            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) {
+                TS v = hr[h];
+                if constexpr (FIX) {
+                    // k_density_fix's order: ring and partial-plane entry of each
+                    // neighbour, then the own partial-plane entry (+0 off the chunk ends)
+                    TS a = TS(0);
+                    if (chunk_end) {
+                        const int ch_w = xw / xc, cs = k == 0 ? (ch_w + nch - 1) % nch : (ch_w + 1) % nch;
+                        const long long ub = ((long long)cs * nty * ntz * 2 + (k == 0 ? 1 : 0)) * AP;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            const TS c = cbuf[ub + co[h][i]];
+                            a += fv[h][i];
+                            a += ro[h][i] >= 0 ? c : TS(0);
+                        }
+                        a += cbuf[ub + co[h][3]];
+                    } else {
+                        a += fv[h][0];
+                        a += fv[h][1];
+                        a += fv[h][2];
+                    }
+                    v = v + a;
+                    if (hown[h] && xg >= xa && xg < xb) rho_out[(long long)xg * P.ny * P.nz + hoff[h]] = v;
+                }
+                s_rho[slot][threadIdx.x + h * TPB] = v;
+            }
+    };
+    for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k, hA, fA); store_plane(k + MH, xa + k, hA, fA); }
+    fetch_plane(xa + MH, hA, fA);
+    fetch_plane(xa + MH + 1, hB, fB);
+    const int base = (ty + MH) * MLZ + tz + MH;
+    const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
+    auto plane_step = [&](int x, TS *hr, TS (*fv)[3]) {
+        __syncthreads();
+        store_plane((x - xa + 2 * MH) % 7, x + MH, hr, fv);   // plane x+3
+        __syncthreads();
+        fetch_plane(x + 2 + MH, hr, fv);                      // plane x+5
+        const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
+        if (active) {
+            int slot[7];
+#pragma unroll
+            for (int dd = 0; dd < 7; ++dd) slot[dd] = (x - xa + dd) % 7;
+            mu[I] = mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + 3]][base + dy * MLZ + dz]; });
+        } else if (rho_solid && inr) {
+            rho_solid[I] = s_rho[(x - xa + 3) % 7][base];
+        }
+    };
+    int x = xa;
+    for (; x + 1 < xb; x += 2) {
+        plane_step(x, hA, fA);
+        plane_step(x + 1, hB, fB);
+    }
+    if (x < xb) plane_step(x, hA, fA);
+}
+
+// With RS the kernel also sums the next step's density: every post-collision
+// value (as stored) is added to the node that will pull it (MacroCalculate,
+// FluidCuda.cu:351-372, summed on the emitting side).  Populations along x stay
+// in the thread's column (registers); in-plane shifts go to an LDS accumulator
+// ring of planes x-1, x, x+1 with a one-node border.  A plane is complete once
+// the plane after it has collided: its tile nodes are stored to rho_next (the
+// sums over this tile and chunk), its border ring to rbuf, and the two partial
+// planes outside the chunk (xa-1, xb) to cbuf.  k_density_fix then adds the
+// rings of the neighbouring tiles and the partial planes of the neighbouring
+// chunks, in a fixed order, with no atomics to global memory.  Requires
+// ny % MTY == nz % MTZ == nx % P.xc == 0 (fuse_geometry).
 template <class TF, class TS, class TC, bool MV, bool RS, bool UR>
 __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
@@ -575,7 +653,9 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
 
     for (int x = xa; x < xb; ++x) {
         const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
-        __syncthreads();                       // every thread is done with plane x-4
+        // every thread is done with plane x-4: with RS the mailbox barrier of the
+        // previous plane (after its arithmetic) already says so
+        if (!RS) __syncthreads();
         store_plane((x - xa + 2 * MH) % 7);    // plane x+3
         __syncthreads();
         TC f[Q];
@@ -1168,14 +1248,15 @@ hipError_t launch_mu(cpmp_ctx *c) {
         } else {
             const int xc = c->dp.xc;   // shorter chunks measured slower (0.077 vs 0.071 ms at 256^3)
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + xc - 1) / xc;
-            hipLaunchKernelGGL(k_mu_march<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
-                               (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc);
+            hipLaunchKernelGGL((k_mu_march<TS, false>), dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
+                               (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc, (const TS *)nullptr,
+                               (const TS *)nullptr, (TS *)nullptr, nch);
         }
         return hipGetLastError();
     });
 }
 
-hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v) {
+hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sums) {
     const bool rs = density_fused(c);
     const hipError_t e = dispatch(c->form, [&](auto fm) {
         using TF = typename decltype(fm)::TF;
@@ -1227,7 +1308,9 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v) {
         return hipGetLastError();
     });
     c->cur ^= 1;
-    if (rs) std::swap(c->rho, c->rho_next);   // the summed density is now current
+    // the summed density is now current (keep_sums: it stays in rho_next for the
+    // next mu kernel to complete)
+    if (rs && !keep_sums) std::swap(c->rho, c->rho_next);
     return e;
 }
 
@@ -1262,6 +1345,49 @@ hipError_t launch_step(cpmp_ctx *c, bool want_mv) {
     if ((e = launch_mu(c)) != hipSuccess) return e;
     if ((e = launch_collide(c, want_mv)) != hipSuccess) return e;
     return launch_macro(c);
+}
+
+// Periodic FP32 lattices with the density summed by the collide kernel: within a
+// run of steps the completion of each step's density (k_density_fix) is folded
+// into the next step's mu kernel (k_mu_march<float, true>); the last step of the
+// run completes its density as usual, so the state between runs is the same.
+// CPMP_DEFER=0 turns this off.
+static bool fix_deferrable(const cpmp_ctx *c) {
+    const char *env = std::getenv("CPMP_DEFER");
+    const bool off = env && std::strcmp(env, "0") == 0;
+    return !off && density_fused(c) && c->form == CPMP_FORM_FP32 && c->dp.lo == 0 && c->dp.hi == 0 &&
+           mu_variant() != 1 && c->dp.xc >= 2;
+}
+
+// the previous collide's tile sums (rho_next) completed into rho, and mu from them
+static hipError_t launch_mu_fix(cpmp_ctx *c) {
+    const int xc = c->dp.xc, ntz = c->p.nz / MTZ, nty = c->p.ny / MTY, nch = c->p.nx / xc;
+    hipLaunchKernelGGL((k_mu_march<float, true>), dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
+                       (const float *)c->rho_next, (float *)c->mu, ntz, nty, (float *)nullptr, xc,
+                       (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch);
+    return hipGetLastError();
+}
+
+hipError_t launch_steps(cpmp_ctx *c, int nsteps) {
+    const bool defer = fix_deferrable(c);
+    hipError_t e;
+    for (int n = 0; n < nsteps; ++n) {
+        const bool last = n == nsteps - 1;
+        if (defer && n > 0) {
+            if ((e = launch_mu_fix(c)) != hipSuccess) return e;
+        } else {
+            if ((e = launch_wall(c)) != hipSuccess) return e;
+            if ((e = launch_mu(c)) != hipSuccess) return e;
+        }
+        if (defer && !last) {
+            if ((e = launch_collide(c, false, nullptr, true)) != hipSuccess) return e;
+        } else {
+            if ((e = launch_collide(c, last)) != hipSuccess) return e;
+            if ((e = launch_macro(c)) != hipSuccess) return e;
+        }
+        ++c->step;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_init_reference(cpmp_ctx *c, double radius, double width, double rg, double rl, double solid) {

@@ -156,3 +156,34 @@ def test_tiled_layout_fields_checkpoints_and_repeatability(cpmp, tmp_path):
     assert np.array_equal(a.get("f"), c.get("f"))
     for g in (a, b, c):
         g.close()
+
+
+PERIODIC_TILED = {
+    # 4 chunks x 2 y tiles x 2 z tiles, periodic: several chunk ends and tile borders
+    "tiled_rke_128x16x64": dict(dims=(128, 16, 64), eos="RKE", tr=0.60, nwall=0,
+                                spheres=[(40, 8, 31.5, 10), (88, 8, 32, 10)], width=5.0),
+}
+GEOMETRIES.update(PERIODIC_TILED)
+
+
+@pytest.mark.parametrize("name", ["tiled_cse_32x8x32", "tiled_rke_32x16x96", "tiled_rke_128x16x64"])
+def test_completion_folded_into_mu_is_bitwise_the_same(cpmp, name):
+    """periodic FP32 lattices: within a run of steps the density completion is done
+    by the next step's mu kernel instead of k_density_fix; the state after every
+    run (densities, populations, mu) is bit-identical to running the completion
+    pass every step (CPMP_DEFER=0)"""
+    a = make(cpmp, name, "fp32")
+    os.environ["CPMP_DEFER"] = "0"
+    try:
+        b = make(cpmp, name, "fp32")
+        for n in (1, 5, 12):
+            os.environ["CPMP_DEFER"] = "0"
+            b.step(n)
+            os.environ.pop("CPMP_DEFER", None)
+            a.step(n)
+            for field in ("rho", "mu", "f"):
+                assert np.array_equal(a.get(field), b.get(field)), (n, field)
+    finally:
+        os.environ.pop("CPMP_DEFER", None)
+    a.close()
+    b.close()

@@ -325,6 +325,100 @@ __global__ void __launch_bounds__(TPB) k_adj_mu(const DevParams P, const TT *__r
     if (threadIdx.x == 0) kapart[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] += s;
 }
 
+// The same transposed chemical potential as a march along x over the tiles of
+// k_adj_stencil_march: the fluid-masked mu cotangent and rho (tape) of planes
+// x-H..x+H with an in-plane halo of H (H = 2 for the 5-point Laplacian, 3 for the
+// 7-point one) sit in an LDS ring.  Same arithmetic per node; the kappa partial
+// sums are accumulated over the chunk and added once per workgroup.
+template <class TT, int H>
+__global__ void __launch_bounds__(TPB) k_adj_mu_march(const DevParams P, const TT *__restrict__ rho0,
+                                                      const double *__restrict__ mub, double *__restrict__ rb_new,
+                                                      double *__restrict__ kapart, int ntz, int nty, int xc) {
+    constexpr int R = 2 * H + 1, EY = kTileY + 2 * H, EZ = kTileZ + 2 * H, EA = EY * EZ;
+    constexpr int SL = (EA + TPB - 1) / TPB;
+    __shared__ double sm[R][EA], sr[R][EA];
+    const long long plane = (long long)P.ny * P.nz;
+    const int tid = threadIdx.x, tz = tid % kTileZ, ty = tid / kTileZ;
+    int b = blockIdx.x;
+    const int zt = b % ntz;
+    b /= ntz;
+    const int yt = b % nty, ch = b / nty;
+    const int z0 = zt * kTileZ, y0 = yt * kTileY, xa = ch * xc, xb = min(xa + xc, P.nx);
+    const int y = y0 + ty, z = z0 + tz;
+    const bool own = y < P.ny && z < P.nz;
+    long long off[SL];
+    int pos[SL];
+    bool sol[SL];
+#pragma unroll
+    for (int r = 0; r < SL; ++r) {
+        const int i = tid + TPB * r;
+        pos[r] = i < EA ? i : -1;
+        const int ii = i < EA ? i : 0;
+        const int gz = wrapc(z0 - H + ii % EZ, P.nz);
+        off[r] = (long long)wrapc(y0 - H + ii / EZ, P.ny) * P.nz + gz;
+        sol[r] = is_solid(P, gz);
+    }
+    double pm[SL], pr[SL];
+    auto fetch = [&](int xp) {   // wraps past the chunk (loads unused then)
+        const long long base = (long long)wrapc(xp, P.nx) * plane;
+#pragma unroll
+        for (int r = 0; r < SL; ++r) {
+            pm[r] = mub[base + off[r]];
+            pr[r] = (double)rho0[base + off[r]];
+        }
+    };
+    auto store = [&](int slot) {
+#pragma unroll
+        for (int r = 0; r < SL; ++r)
+            if (pos[r] >= 0) {
+                sm[slot][pos[r]] = sol[r] ? 0.0 : pm[r];   // the Laplacian sees the fluid cotangent only
+                sr[slot][pos[r]] = pr[r];
+            }
+    };
+    for (int k = -H; k < H; ++k) { fetch(xa + k); store(k + H); }
+    fetch(xa + H);
+    const int c0 = (ty + H) * EZ + tz + H;
+    double kap = 0;
+    for (int x = xa; x < xb; ++x) {
+        __syncthreads();                // plane x-H-1 is no longer read
+        store((x - xa + 2 * H) % R);    // plane x+H
+        __syncthreads();
+        fetch(x + H + 1);
+        if (own) {
+            int slot[R];
+#pragma unroll
+            for (int d = 0; d < R; ++d) slot[d] = (x - xa + d) % R;
+            auto mm = [&](int dx, int dy, int dz) { return sm[slot[dx + H]][c0 + dy * EZ + dz]; };
+            auto rr = [&](int dx, int dy, int dz) { return sr[slot[dx + H]][c0 + dy * EZ + dz]; };
+            auto lap = [&](auto &&v) {
+                if constexpr (H == 3) {
+                    auto ax = [&](int ux, int uy, int uz) {
+                        return 2 * v(-3 * ux, -3 * uy, -3 * uz) - 27 * v(-2 * ux, -2 * uy, -2 * uz) + 270 * v(-ux, -uy, -uz)
+                               + 270 * v(ux, uy, uz) - 27 * v(2 * ux, 2 * uy, 2 * uz) + 2 * v(3 * ux, 3 * uy, 3 * uz);
+                    };
+                    return (-v(0, 0, 0) * 490 * 3 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 180;
+                } else {
+                    auto ax = [&](int ux, int uy, int uz) {
+                        return -v(-2 * ux, -2 * uy, -2 * uz) + v(-ux, -uy, -uz) * 16 + v(ux, uy, uz) * 16
+                               - v(2 * ux, 2 * uy, 2 * uz);
+                    };
+                    return (-v(0, 0, 0) * 90 + ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) / 12;
+                }
+            };
+            const long long I = (long long)x * plane + (long long)y * P.nz + z;
+            double acc = -P.Ka * lap(mm);
+            if (!is_solid(P, z)) {
+                const double r = rr(0, 0, 0), mu_b = mm(0, 0, 0);
+                acc += P.K2s * dmu_eos(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho(r)) * clamp_drho(r) * mu_b;
+                kap += -mu_b * lap(rr);
+            }
+            rb_new[I] += acc;
+        }
+    }
+    const double sk = block_sum(kap);
+    if (threadIdx.x == 0) kapart[blockIdx.x] += sk;
+}
+
 // transposed wall fill, one layer pass (run deepest pass first).  The deepest
 // pass also adds the wall-field gradients (solid mu = mu_w of the column) when
 // mub is given.
@@ -451,10 +545,26 @@ struct AdjPlan {
     size_t per_ck = 0;       // bytes of one checkpoint: populations + rho in the storage form
     size_t per_tape = 0;     // bytes of one tape entry: 10 values of the storage scalar type per node
     size_t nbuf = 0;         // doubles of the cotangent and work buffers
+    long long nkap = 0;      // kappa partial-sum slots
     size_t fixed = 0;        // bytes besides checkpoints and tape
     size_t total = 0;        // device bytes allocated by the call
     long long recomputed = 0;
 };
+
+// Tiles and chunks of the x-march reverse kernels (k_adj_stencil_march,
+// k_adj_mu_march): the longest chunk of 32, 16 or 8 planes that gives at least
+// 1024 workgroups, else 8 (shorter chunks re-read too many halo planes: measured
+// slower than the one-thread-per-node kernels at 96x96x64, which chunks of 8 beat);
+// CPMP_ADJ_XC sets a chunk length.
+void adj_march_geometry(const DevParams &P, int &ntz, int &nty, int &xc, unsigned &blocks) {
+    ntz = (P.nz + kTileZ - 1) / kTileZ;
+    nty = (P.ny + kTileY - 1) / kTileY;
+    xc = 8;
+    for (int cand = 32; cand >= 8; cand /= 2)
+        if ((long long)((P.nx + cand - 1) / cand) * ntz * nty >= 1024) { xc = cand; break; }
+    if (const char *xe = std::getenv("CPMP_ADJ_XC")) xc = std::max(1, std::min(64, std::atoi(xe)));
+    blocks = (unsigned)((P.nx + xc - 1) / xc * ntz * nty);
+}
 
 int sub_count(int L, int S) { return (L + S - 1) / S; }
 
@@ -475,7 +585,11 @@ int make_plan(cpmp_ctx *c, const cpmp_adj_desc *d, AdjPlan &q) {
     q.C = std::min(d->checkpoint_every, d->nsteps);
     q.per_ck = (size_t)N * (19 * c->pop_bytes + c->scal_bytes);
     q.per_tape = 10 * (size_t)N * c->scal_bytes;
-    q.nbuf = (size_t)N * (19 * 2 + 3 + 6 + 1) + std::max(nb, nbg) + nbg + 2 * NXY + 4;
+    int ntz_a, nty_a, xc_a;
+    unsigned mblocks;
+    adj_march_geometry(c->dp, ntz_a, nty_a, xc_a, mblocks);
+    q.nkap = std::max<long long>(nbg, mblocks);   // kappa partial sums: one slot per workgroup
+    q.nbuf = (size_t)N * (19 * 2 + 3 + 6 + 1) + std::max(nb, nbg) + q.nkap + 2 * NXY + 4;
     // buffers, the final state in the storage form, the target
     q.fixed = q.nbuf * sizeof(double) + q.per_ck + (size_t)N * sizeof(double);
     size_t fr = 0, tot = 0;
@@ -556,17 +670,14 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const DevParams &P = c->dp;
     const dim3 gwt((P.nx + kWallTile - 1) / kWallTile, (P.ny + kWallTile - 1) / kWallTile, 2), bwt(kWallTile * kWallTile);
     const int passes = std::max(P.lo, P.hi);
-    // transposed force stencils: x march over tiles and chunks of planes (at least
-    // 1024 workgroups), when the chunks are at least 8 planes long (the 6 halo
-    // planes per chunk cost less than the cache misses of the one-thread-per-node
-    // kernel); CPMP_ADJ_STENCIL=v1 / march forces either
-    const int ntz_a = (P.nz + kTileZ - 1) / kTileZ, nty_a = (P.ny + kTileY - 1) / kTileY;
-    int xc_a = 2;
-    for (int cand = 32; cand >= 2; cand /= 2)
-        if ((long long)((P.nx + cand - 1) / cand) * ntz_a * nty_a >= 1024) { xc_a = cand; break; }
+    int ntz_a, nty_a, xc_a;
+    unsigned march_blocks;
+    adj_march_geometry(P, ntz_a, nty_a, xc_a, march_blocks);
     const char *sv = std::getenv("CPMP_ADJ_STENCIL");
-    const bool march = sv && std::strcmp(sv, "v1") == 0 ? false : (sv && std::strcmp(sv, "march") == 0) || xc_a >= 8;
-    const unsigned march_blocks = (unsigned)((P.nx + xc_a - 1) / xc_a * ntz_a * nty_a);
+    const bool march = !(sv && std::strcmp(sv, "v1") == 0);
+    // the chemical-potential march pays only with long chunks (measured: 5 % less
+    // reverse time at 512x512x256, chunks of 32; 2 % more at 96x96x64, chunks of 8)
+    const bool march_mu = march && (xc_a >= 16 || (sv && std::strcmp(sv, "march") == 0));
     const int64_t step0 = c->step;
     const size_t fbytes = 19 * (size_t)N * c->pop_bytes, rbytes = (size_t)N * c->scal_bytes;
     // tape entry of step n, in the storage scalar type (FP64 or FP32): rho^n after
@@ -589,7 +700,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             fbA = p; p += 19 * N; fbB = p; p += 19 * N;
             rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
             Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
-            part = p; p += std::max<long long>(nb, nbg); kpart = p; p += nbg;
+            part = p; p += std::max<long long>(nb, nbg); kpart = p; p += q.nkap;
             gl = p; p += NXY; gh = p; p += NXY; acc = p; p += 4;
         }
         AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
@@ -636,7 +747,14 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
                     } else
                         hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf,
                                            (const double *)Wf, (const double *)rbl, mub, rbn);
-                    hipLaunchKernelGGL(k_adj_mu<TT>, node_grid(P), dim3(TPB), 0, s, P, tt, (const double *)mub, rbn, kpart);
+                    if (!march_mu)
+                        hipLaunchKernelGGL(k_adj_mu<TT>, node_grid(P), dim3(TPB), 0, s, P, tt, (const double *)mub, rbn, kpart);
+                    else if (P.lap_points == 7)
+                        hipLaunchKernelGGL((k_adj_mu_march<TT, 3>), dim3(march_blocks), dim3(TPB), 0, s, P, tt,
+                                           (const double *)mub, rbn, kpart, ntz_a, nty_a, xc_a);
+                    else
+                        hipLaunchKernelGGL((k_adj_mu_march<TT, 2>), dim3(march_blocks), dim3(TPB), 0, s, P, tt,
+                                           (const double *)mub, rbn, kpart, ntz_a, nty_a, xc_a);
                 };
                 if (sb == sizeof(double)) tape_kernels((const double *)t0);
                 else tape_kernels((const float *)t0);
@@ -700,7 +818,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             }
         }
     }
-    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)kpart, nbg, acc + 1);
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(TPB), 0, s, (const double *)kpart, q.nkap, acc + 1);
     AHIP(hipGetLastError());
     AHIP(hipMemcpyAsync(acc_h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     if (g_lo) AHIP(hipMemcpyAsync(g_lo, gl, NXY * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -75,11 +75,13 @@ __global__ void __launch_bounds__(TPB) k_adj_J(const DevParams P, const TT *__re
 // without rbar is local to y and was written by the previous reverse step as fh_in
 // (fbar_q + ubar.(c_q - u)/rho), so each pulled value costs two loads.  This kernel
 // writes the same folded form for step n: fh_out_q = fbar_q + ubar.(c_q - u^n)/rho^n.
-template <class TT>
-__global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const double *__restrict__ fh_in,
+// TT: the tape's type; TH: the storage type of the folded population cotangents
+// (FP64; FP32 for the adjoint of an FP32 run), arithmetic in FP64.
+template <class TT, class TH>
+__global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const TH *__restrict__ fh_in,
                                                      const double *__restrict__ rb_in,
                                                      const TT *__restrict__ rho0, const TT *__restrict__ u0,
-                                                     const TT *__restrict__ grads, double *__restrict__ fh_out,
+                                                     const TT *__restrict__ grads, TH *__restrict__ fh_out,
                                                      double *__restrict__ rb_loc,
                                                      double *__restrict__ Pf, double *__restrict__ Wf) {
     const NodeIdx nd = node_from_grid(P);
@@ -91,7 +93,7 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
         rb_loc[I] = 0;
         for (int a = 0; a < 3; ++a) { Pf[a * N + I] = 0; Wf[a * N + I] = 0; }
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fh_out[(long long)q * N + I] = 0;
+        for (int q = 0; q < Q; ++q) fh_out[(long long)q * N + I] = TH(0);
         return;
     }
     int xs[7], ys[7], zs[7];
@@ -104,7 +106,7 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
         const bool bounce = is_solid(P, tz);
         const long long y = bounce ? I : ((long long)xs[3 + cx(q)] * P.ny + ys[3 + cy(q)]) * P.nz + tz;
         const int qq = bounce ? opp(q) : q;   // bounced back: pulled by this node as opp(q), c_opp(q) = -c_q
-        D[q] = fh_in[(long long)qq * N + y] + rb_in[y];
+        D[q] = (double)fh_in[(long long)qq * N + y] + rb_in[y];
     }
     double m[Q], mb[Q], b[Q];
     moments(D, m);
@@ -136,7 +138,7 @@ __global__ void __launch_bounds__(TPB) k_adj_collide(const DevParams P, const do
         const double ubu = ubx * ux + uby * uy + ubz * uz;
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            fh_out[(long long)q * N + I] = fbar[q] + (cx(q) * ubx + cy(q) * uby + cz(q) * ubz - ubu);
+            fh_out[(long long)q * N + I] = (TH)(fbar[q] + (cx(q) * ubx + cy(q) * uby + cz(q) * ubz - ubu));
     }
     double wx = Fbx / 3, wy = Fby / 3, wz = Fbz / 3;
     if (P.qp_on) {
@@ -589,7 +591,9 @@ int make_plan(cpmp_ctx *c, const cpmp_adj_desc *d, AdjPlan &q) {
     unsigned mblocks;
     adj_march_geometry(c->dp, ntz_a, nty_a, xc_a, mblocks);
     q.nkap = std::max<long long>(nbg, mblocks);   // kappa partial sums: one slot per workgroup
-    q.nbuf = (size_t)N * (19 * 2 + 3 + 6 + 1) + std::max(nb, nbg) + q.nkap + 2 * NXY + 4;
+    // folded population cotangents in the storage precision of the form
+    const size_t hb = c->form == CPMP_FORM_FP64 ? sizeof(double) : sizeof(float);
+    q.nbuf = 2 * ((19 * (size_t)N * hb + 7) / 8) + (size_t)N * (3 + 6 + 1) + std::max(nb, nbg) + q.nkap + 2 * NXY + 4;
     // buffers, the final state in the storage form, the target
     q.fixed = q.nbuf * sizeof(double) + q.per_ck + (size_t)N * sizeof(double);
     size_t fr = 0, tot = 0;
@@ -680,13 +684,15 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     const bool march_mu = march && (xc_a >= 16 || (sv && std::strcmp(sv, "march") == 0));
     const int64_t step0 = c->step;
     const size_t fbytes = 19 * (size_t)N * c->pop_bytes, rbytes = (size_t)N * c->scal_bytes;
+    const size_t hb = c->form == CPMP_FORM_FP64 ? sizeof(double) : sizeof(float);   // population cotangents
     // tape entry of step n, in the storage scalar type (FP64 or FP32): rho^n after
     // the wall fill (N), then u^n (3N), grad mu^n (3N) and grad rho^n (3N) as the
     // forward collide evaluated them
     const size_t sb = c->scal_bytes;
     char *ckpt = nullptr, *subck = nullptr, *fin = nullptr, *tape = nullptr;
     double *buf = nullptr, *tgt = nullptr;
-    double *fbA, *fbB, *rb, *rbn, *rbl, *Pf, *Wf, *mub, *part, *kpart, *gl, *gh, *acc;
+    void *fbA, *fbB;
+    double *rb, *rbn, *rbl, *Pf, *Wf, *mub, *part, *kpart, *gl, *gh, *acc;
     double acc_h[2] = {0, 0};
     {
         AHIP(hipMalloc((void **)&ckpt, (size_t)std::max(q.nseg, 1) * q.per_ck));
@@ -697,7 +703,8 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
         {
             double *p = buf;
-            fbA = p; p += 19 * N; fbB = p; p += 19 * N;
+            // folded population cotangents: two sets of 19 N values of hb bytes
+            fbA = p; p += (19 * N * hb + 7) / 8; fbB = p; p += (19 * N * hb + 7) / 8;
             rb = p; p += N; rbn = p; p += N; rbl = p; p += N;
             Pf = p; p += 3 * N; Wf = p; p += 3 * N; mub = p; p += N;
             part = p; p += std::max<long long>(nb, nbg); kpart = p; p += q.nkap;
@@ -737,8 +744,14 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
                 const char *t0 = tape + k * q.per_tape;
                 auto tape_kernels = [&](auto *tt) {
                     using TT = std::remove_const_t<std::remove_pointer_t<decltype(tt)>>;
-                    hipLaunchKernelGGL(k_adj_collide<TT>, node_grid(P), dim3(TPB), 0, s, P, (const double *)fbA,
-                                       (const double *)rb, tt, tt + N, tt + 4 * N, fbB, rbl, Pf, Wf);
+                    if (hb == sizeof(double))
+                        hipLaunchKernelGGL((k_adj_collide<TT, double>), node_grid(P), dim3(TPB), 0, s, P,
+                                           (const double *)fbA, (const double *)rb, tt, tt + N, tt + 4 * N, (double *)fbB,
+                                           rbl, Pf, Wf);
+                    else
+                        hipLaunchKernelGGL((k_adj_collide<TT, float>), node_grid(P), dim3(TPB), 0, s, P,
+                                           (const float *)fbA, (const double *)rb, tt, tt + N, tt + 4 * N, (float *)fbB,
+                                           rbl, Pf, Wf);
                     if (march) {
                         hipLaunchKernelGGL(k_adj_stencil_march<false>, dim3(march_blocks), dim3(TPB), 0, s, P,
                                            (const double *)Pf, (const double *)nullptr, mub, ntz_a, nty_a, xc_a);

@@ -226,9 +226,9 @@ int cpmp_write_plt(cpmp_ctx *ctx, const char *path, const char *title, int decim
 /* J = 1/2 sum_fluid (rho^N - target)^2 after d->nsteps steps from the current state.
    Returns dJ/d mu_w(bottom), dJ/d mu_w(top) (nx*ny each, may be NULL) and dJ/dKa.
    The state is left at step N of the forward run.  FP64 and FP32 forms: the forward
-   run (and its recomputation) is the context's own step; the reverse sweep runs in
-   FP64 on the tape, so in the FP32 form it is the FP64 adjoint evaluated along the
-   FP32 trajectory. */
+   run (and its recomputation) is the context's own step; the reverse sweep computes
+   in FP64 on the tape, so in the FP32 form it is the FP64 adjoint evaluated along the
+   FP32 trajectory (its population cotangents are stored in FP32 between steps). */
 int cpmp_adjoint(cpmp_ctx *ctx, const cpmp_adj_desc *d, double *J, double *g_mu_lo,
                  double *g_mu_hi, double *g_ka);
 /* The schedule cpmp_adjoint would take for d, without running it (CPMP_ENOMEM when

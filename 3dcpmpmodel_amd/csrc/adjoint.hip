@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const do
 }
 
 // The same transposed stencils as a march along x, one field triple per launch
-// (Pf -> mub, Wf -> rb_new).  A workgroup owns a tile of kTileY x kTileZ columns
+// (Pf -> mub, Wf -> rb_new).  A workgroup owns a tile of kAdjTY x kAdjTZ columns
 // and a chunk [xa, xb) of planes; it streams the planes xa-3 .. xb+2 of the
 // triple (with a halo of 3 in y and z) through LDS one at a time, and each thread
 // adds the contributions of the plane in LDS to the seven output planes that
@@ -202,10 +202,11 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil(const DevParams P, const do
 //   sum_s c_s [2 Ay(y+s,z) + 2 Az(y,z+s) + (Ay+Az)(y+s,z+s) + (Ay-Az)(y+s,z-s)],
 // the sum of grad_divergence above (summed in another order).
 // SUB: out = base - sum / 360 (rb_new from rb_loc); otherwise out = sum / 360.
+constexpr int kAdjTY = 8, kAdjTZ = 32;   // tile of the reverse march kernels (one thread per column)
 constexpr int kAdjHalo = 3;
-constexpr int kAdjEY = kTileY + 2 * kAdjHalo, kAdjEZ = kTileZ + 2 * kAdjHalo, kAdjEA = kAdjEY * kAdjEZ;
+constexpr int kAdjEY = kAdjTY + 2 * kAdjHalo, kAdjEZ = kAdjTZ + 2 * kAdjHalo, kAdjEA = kAdjEY * kAdjEZ;
 constexpr int kAdjSlots = (kAdjEA + TPB - 1) / TPB;
-static_assert(kTileY * kTileZ == TPB, "one thread per tile column");
+static_assert(kAdjTY * kAdjTZ == TPB, "one thread per tile column");
 
 template <bool SUB>
 __global__ void __launch_bounds__(TPB) k_adj_stencil_march(const DevParams P, const double *__restrict__ A,
@@ -213,12 +214,12 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil_march(const DevParams P, co
                                                            int ntz, int nty, int xc) {
     __shared__ double sf[3][kAdjEA];   // x, y, z components of one plane
     const long long N = P.N, plane = (long long)P.ny * P.nz;
-    const int tid = threadIdx.x, tz = tid % kTileZ, ty = tid / kTileZ;
+    const int tid = threadIdx.x, tz = tid % kAdjTZ, ty = tid / kAdjTZ;
     int b = blockIdx.x;
     const int zt = b % ntz;
     b /= ntz;
     const int yt = b % nty, ch = b / nty;
-    const int z0 = zt * kTileZ, y0 = yt * kTileY, xa = ch * xc, xb = min(xa + xc, P.nx);
+    const int z0 = zt * kAdjTZ, y0 = yt * kAdjTY, xa = ch * xc, xb = min(xa + xc, P.nx);
     const int y = y0 + ty, z = z0 + tz;
     const bool own = y < P.ny && z < P.nz;
     // LDS slots this thread fills: position i = tid + TPB r of the haloed tile
@@ -336,16 +337,16 @@ template <class TT, int H>
 __global__ void __launch_bounds__(TPB) k_adj_mu_march(const DevParams P, const TT *__restrict__ rho0,
                                                       const double *__restrict__ mub, double *__restrict__ rb_new,
                                                       double *__restrict__ kapart, int ntz, int nty, int xc) {
-    constexpr int R = 2 * H + 1, EY = kTileY + 2 * H, EZ = kTileZ + 2 * H, EA = EY * EZ;
+    constexpr int R = 2 * H + 1, EY = kAdjTY + 2 * H, EZ = kAdjTZ + 2 * H, EA = EY * EZ;
     constexpr int SL = (EA + TPB - 1) / TPB;
     __shared__ double sm[R][EA], sr[R][EA];
     const long long plane = (long long)P.ny * P.nz;
-    const int tid = threadIdx.x, tz = tid % kTileZ, ty = tid / kTileZ;
+    const int tid = threadIdx.x, tz = tid % kAdjTZ, ty = tid / kAdjTZ;
     int b = blockIdx.x;
     const int zt = b % ntz;
     b /= ntz;
     const int yt = b % nty, ch = b / nty;
-    const int z0 = zt * kTileZ, y0 = yt * kTileY, xa = ch * xc, xb = min(xa + xc, P.nx);
+    const int z0 = zt * kAdjTZ, y0 = yt * kAdjTY, xa = ch * xc, xb = min(xa + xc, P.nx);
     const int y = y0 + ty, z = z0 + tz;
     const bool own = y < P.ny && z < P.nz;
     long long off[SL];
@@ -559,8 +560,8 @@ struct AdjPlan {
 // slower than the one-thread-per-node kernels at 96x96x64, which chunks of 8 beat);
 // CPMP_ADJ_XC sets a chunk length.
 void adj_march_geometry(const DevParams &P, int &ntz, int &nty, int &xc, unsigned &blocks) {
-    ntz = (P.nz + kTileZ - 1) / kTileZ;
-    nty = (P.ny + kTileY - 1) / kTileY;
+    ntz = (P.nz + kAdjTZ - 1) / kAdjTZ;
+    nty = (P.ny + kAdjTY - 1) / kAdjTY;
     xc = 8;
     for (int cand = 32; cand >= 8; cand /= 2)
         if ((long long)((P.nx + cand - 1) / cand) * ntz * nty >= 1024) { xc = cand; break; }

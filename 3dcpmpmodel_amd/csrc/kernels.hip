@@ -322,6 +322,7 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
 // tile instead of 54 times per stencil; the populations of plane x are loaded
 // (coalesced) before the ring update so their latency overlaps the barrier.
 constexpr int MTY = TILE_Y, MTZ = TILE_Z, MH = 3, MXC = 32;   // MXC: longest chunk (P.xc <= MXC)
+constexpr int MT = MTY * MTZ;   // threads of a march workgroup: one per tile column
 constexpr int MLY = MTY + 2 * MH, MLZ = MTZ + 2 * MH, MLP = MLY * MLZ;
 
 template <class TS, class TC>
@@ -374,7 +375,7 @@ __device__ __forceinline__ int ring_index(int ly, int lz) {
 // density of the workgroup's own tile and chunk is written to rho_out.  This
 // replaces the separate completion pass.
 template <class TS, bool FIX>
-__global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
+__global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
                                                   int ntz, int nty, TS *__restrict__ rho_solid, int xc,
                                                   const TS *__restrict__ rbuf, const TS *__restrict__ cbuf,
                                                   TS *__restrict__ rho_out, int nch) {
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
     const int y = y0 + ty, z = z0 + tz;
     const bool inr = (y < P.ny) && (z < P.nz);
     const bool active = inr && !is_solid(P, z);
-    constexpr int HE = (MLP + TPB - 1) / TPB;
+    constexpr int HE = (MLP + MT - 1) / MT;
     int hoff[HE], hy[HE], hz[HE];
     bool hown[HE];
     // FIX: in-plane part of the ring index of the (up to three) neighbouring tiles
@@ -403,7 +404,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
     const long long plane_ring = (long long)nty * ntz * ARING;
 #pragma unroll
     for (int h = 0; h < HE; ++h) {
-        const int e = threadIdx.x + h * TPB;
+        const int e = threadIdx.x + h * MT;
         const int ly = e / MLZ, lz = e - ly * MLZ;
         hy[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) : 0;
         hz[h] = e < MLP ? wrapc(z0 - MH + lz, P.nz) : 0;
@@ -442,7 +443,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
         const TS *rp = FIX ? rbuf + (long long)xg * plane_ring : nullptr;   // ring entries of plane xg
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) {
+            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) {
                 hr[h] = rho[pb + hoff[h]];
                 if constexpr (FIX) {
 #pragma unroll
@@ -455,7 +456,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
         const bool chunk_end = k == 0 || k == xc - 1;   // partial-plane terms too (uniform)
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) {
+            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) {
                 TS v = hr[h];
                 if constexpr (FIX) {
                     // k_density_fix's order: ring and partial-plane entry of each
@@ -479,7 +480,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
                     v = v + a;
                     if (hown[h] && xg >= xa && xg < xb) rho_out[(long long)xg * P.ny * P.nz + hoff[h]] = v;
                 }
-                s_rho[slot][threadIdx.x + h * TPB] = v;
+                s_rho[slot][threadIdx.x + h * MT] = v;
             }
     };
     for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k, hA, fA); store_plane(k + MH, xa + k, hA, fA); }
@@ -522,7 +523,7 @@ __global__ void __launch_bounds__(TPB) k_mu_march(const DevParams P, const TS *_
 // chunks, in a fixed order, with no atomics to global memory.  Requires
 // ny % MTY == nz % MTZ == nx % P.xc == 0 (fuse_geometry).
 template <class TF, class TS, class TC, bool MV, bool RS, bool UR>
-__global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const TF *__restrict__ fin,
+__global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
                                                        TS *__restrict__ uout, int ntz, int nty,
@@ -530,7 +531,7 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
                                                        TS *__restrict__ cbuf, int remap, const TS *__restrict__ rref) {
     __shared__ TS s_rho[7][MLP];
     __shared__ TS s_mu[7][MLP];
-    __shared__ TC s_mail[RS ? Q - 3 : 1][RS ? TPB : 1];   // in-plane populations 3..18 of plane x
+    __shared__ TC s_mail[RS ? Q - 3 : 1][RS ? MT : 1];   // in-plane populations 3..18 of plane x
     const long long N = P.N;
     // tile order: plain, or (remap) each XCD a contiguous range of tiles --
     // workgroups are dispatched round-robin over the 8 XCDs
@@ -547,13 +548,13 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
     const int y = y0 + ty, z = z0 + tz;
     const bool active = (y < P.ny) && (z < P.nz) && !is_solid(P, z);
 
-    // Halo entries of one plane owned by this thread (e = tid + h*TPB) and their
+    // Halo entries of one plane owned by this thread (e = tid + h*MT) and their
     // in-plane offsets; the plane index is added per x.
-    constexpr int HE = (MLP + TPB - 1) / TPB;
+    constexpr int HE = (MLP + MT - 1) / MT;
     int hoff[HE];
 #pragma unroll
     for (int h = 0; h < HE; ++h) {
-        const int e = threadIdx.x + h * TPB;
+        const int e = threadIdx.x + h * MT;
         const int ly = e / MLZ, lz = e - ly * MLZ;
         hoff[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) * P.nz + wrapc(z0 - MH + lz, P.nz) : 0;
     }
@@ -562,12 +563,12 @@ __global__ void __launch_bounds__(TPB) k_collide_march(const DevParams P, const 
         const long long pb = (long long)wrapc(xg, P.nx) * P.ny * P.nz;
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) { hr[h] = rho[pb + hoff[h]]; hm[h] = mu[pb + hoff[h]]; }
+            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) { hr[h] = rho[pb + hoff[h]]; hm[h] = mu[pb + hoff[h]]; }
     };
     auto store_plane = [&](int slot) {      // registers -> LDS ring slot
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if ((h + 1) * TPB <= MLP || threadIdx.x + h * TPB < MLP) { s_rho[slot][threadIdx.x + h * TPB] = hr[h]; s_mu[slot][threadIdx.x + h * TPB] = hm[h]; }
+            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) { s_rho[slot][threadIdx.x + h * MT] = hr[h]; s_mu[slot][threadIdx.x + h * MT] = hm[h]; }
     };
     for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k); store_plane(k + MH); }
 
@@ -759,7 +760,7 @@ constexpr int ETILE = 2 * MTZ + 2 * (MTY - 2), ITILE = (MTY - 2) * (MTZ - 2);
 constexpr int FIX_ITEMS_MAX = MXC * ETILE + 2 * ITILE;
 
 template <class TS>
-__global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__restrict__ rho, const TS *__restrict__ rbuf,
+__global__ void __launch_bounds__(MT) k_density_fix(const DevParams P, TS *__restrict__ rho, const TS *__restrict__ rbuf,
                                                      const TS *__restrict__ cbuf, int ntz, int nty, int nch) {
     int t = blockIdx.x;
     const int tzi = t % ntz;
@@ -779,14 +780,14 @@ __global__ void __launch_bounds__(TPB) k_density_fix(const DevParams P, TS *__re
         nbase[i] = ((((long long)cn * nty + yy) * ntz + zz) * 2 + 0) * AP;
     }
     // every item of this thread at once (all loads in flight), then the updates
-    constexpr int PER = (FIX_ITEMS_MAX + TPB - 1) / TPB;
+    constexpr int PER = (FIX_ITEMS_MAX + MT - 1) / MT;
     const int items = P.xc * ETILE + 2 * ITILE;
     const long long plane_ring = (long long)nty * ntz * ARING;
     TS add[PER];
     long long at[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int it = threadIdx.x + j * TPB;
+        const int it = threadIdx.x + j * MT;
         add[j] = TS(0);
         at[j] = -1;
         if (it >= items) continue;
@@ -1248,7 +1249,7 @@ hipError_t launch_mu(cpmp_ctx *c) {
         } else {
             const int xc = c->dp.xc;   // shorter chunks measured slower (0.077 vs 0.071 ms at 256^3)
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + xc - 1) / xc;
-            hipLaunchKernelGGL((k_mu_march<TS, false>), dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
+            hipLaunchKernelGGL((k_mu_march<TS, false>), dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0, c->stream, c->dp,
                                (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc, (const TS *)nullptr,
                                (const TS *)nullptr, (TS *)nullptr, nch);
         }
@@ -1290,7 +1291,7 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sum
             auto go = [&](auto mv_t, auto rs_t, auto ur_t) {
                 hipLaunchKernelGGL((k_collide_march<TF, TS, TC, decltype(mv_t)::value, decltype(rs_t)::value,
                                                     decltype(ur_t)::value>),
-                                   grid, dim3(TPB), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb,
+                                   grid, dim3(MT), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb,
                                    cb, remap, ref_buffer<TS>(c));
             };
             using T1 = std::true_type;
@@ -1319,7 +1320,7 @@ hipError_t launch_macro(cpmp_ctx *c) {
         return dispatch(c->form, [&](auto fm) {
             using TS = typename decltype(fm)::TS;
             const int ntz = c->p.nz / MTZ, nty = c->p.ny / MTY, nch = c->p.nx / c->dp.xc;
-            hipLaunchKernelGGL(k_density_fix<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
+            hipLaunchKernelGGL(k_density_fix<TS>, dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0, c->stream, c->dp,
                                (TS *)c->rho, (const TS *)c->rbuf, (const TS *)c->cbuf, ntz, nty, nch);
             return hipGetLastError();
         });
@@ -1362,7 +1363,7 @@ static bool fix_deferrable(const cpmp_ctx *c) {
 // the previous collide's tile sums (rho_next) completed into rho, and mu from them
 static hipError_t launch_mu_fix(cpmp_ctx *c) {
     const int xc = c->dp.xc, ntz = c->p.nz / MTZ, nty = c->p.ny / MTY, nch = c->p.nx / xc;
-    hipLaunchKernelGGL((k_mu_march<float, true>), dim3((unsigned)(ntz * nty * nch)), dim3(TPB), 0, c->stream, c->dp,
+    hipLaunchKernelGGL((k_mu_march<float, true>), dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0, c->stream, c->dp,
                        (const float *)c->rho_next, (float *)c->mu, ntz, nty, (float *)nullptr, xc,
                        (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch);
     return hipGetLastError();

@@ -347,6 +347,48 @@ struct LdsStencil {
     }
 };
 
+// rho and mu of one node side by side in LDS: one 8-byte (16-byte in FP64) read
+// serves both stencils
+template <class TS>
+struct __align__(2 * sizeof(TS)) RhoMu {
+    TS r, m;
+};
+
+// The isotropic gradients of rho and mu (Stencil above) from the RhoMu ring.
+template <class TS, class TC>
+struct LdsStencil2 {
+    const RhoMu<TS> (*a)[MLP];
+    const int *slot;   // ring slot of plane x + dx, dx = -3..3
+    int base;          // (ty + MH) * MLZ + tz + MH
+    __device__ __forceinline__ RhoMu<TS> at(int dx, int dy, int dz) const { return a[slot[dx + 3]][base + dy * MLZ + dz]; }
+    // 6-point differences of rho (dr) and mu (dm) along u
+    __device__ __forceinline__ void dd(int ux, int uy, int uz, TC &dr, TC &dm) const {
+        const RhoMu<TS> m3 = at(-3 * ux, -3 * uy, -3 * uz), m2 = at(-2 * ux, -2 * uy, -2 * uz), m1 = at(-ux, -uy, -uz);
+        const RhoMu<TS> p1 = at(ux, uy, uz), p2 = at(2 * ux, 2 * uy, 2 * uz), p3 = at(3 * ux, 3 * uy, 3 * uz);
+        dr = -(TC)m3.r + TC(9) * (TC)m2.r - TC(45) * (TC)m1.r + TC(45) * (TC)p1.r - TC(9) * (TC)p2.r + (TC)p3.r;
+        dm = -(TC)m3.m + TC(9) * (TC)m2.m - TC(45) * (TC)m1.m + TC(45) * (TC)p1.m - TC(9) * (TC)p2.m + (TC)p3.m;
+    }
+    __device__ __forceinline__ void grad(TC &grx, TC &gry, TC &grz, TC &gmx, TC &gmy, TC &gmz) const {
+        const TC s = TC(1.0 / 360.0);
+        TC rx, mx, ry, my, rz, mz, rxy, mxy, rxmy, mxmy, rxz, mxz, rxmz, mxmz, ryz, myz, rymz, mymz;
+        dd(1, 0, 0, rx, mx);
+        dd(0, 1, 0, ry, my);
+        dd(0, 0, 1, rz, mz);
+        dd(1, 1, 0, rxy, mxy);
+        dd(1, -1, 0, rxmy, mxmy);
+        dd(1, 0, 1, rxz, mxz);
+        dd(1, 0, -1, rxmz, mxmz);
+        dd(0, 1, 1, ryz, myz);
+        dd(0, 1, -1, rymz, mymz);
+        grx = (TC(2) * rx + rxz + rxmz + rxy + rxmy) * s;
+        gry = (TC(2) * ry + ryz + rymz + rxy - rxmy) * s;
+        grz = (TC(2) * rz + rxz - rxmz + ryz - rymz) * s;
+        gmx = (TC(2) * mx + mxz + mxmz + mxy + mxmy) * s;
+        gmy = (TC(2) * my + myz + mymz + mxy - mxmy) * s;
+        gmz = (TC(2) * mz + mxz - mxmz + myz - mymz) * s;
+    }
+};
+
 constexpr int AY = MTY + 2, AZ = MTZ + 2, AP = AY * AZ, ARING = AP - MTY * MTZ;
 
 // accumulator position (ly, lz) of ring entry r: rows ly = 0 and ly = AY-1, then
@@ -529,8 +571,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
                                                        TS *__restrict__ uout, int ntz, int nty,
                                                        TS *__restrict__ rho_next, TS *__restrict__ rbuf,
                                                        TS *__restrict__ cbuf, int remap, const TS *__restrict__ rref) {
-    __shared__ TS s_rho[7][MLP];
-    __shared__ TS s_mu[7][MLP];
+    __shared__ RhoMu<TS> s_rm[7][MLP];
     __shared__ TC s_mail[RS ? Q - 3 : 1][RS ? MT : 1];   // in-plane populations 3..18 of plane x
     const long long N = P.N;
     // tile order: plain, or (remap) each XCD a contiguous range of tiles --
@@ -568,7 +609,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     auto store_plane = [&](int slot) {      // registers -> LDS ring slot
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) { s_rho[slot][threadIdx.x + h * MT] = hr[h]; s_mu[slot][threadIdx.x + h * MT] = hm[h]; }
+            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) s_rm[slot][threadIdx.x + h * MT] = RhoMu<TS>{hr[h], hm[h]};
     };
     for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k); store_plane(k + MH); }
 
@@ -668,11 +709,10 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
             int slot[7];
 #pragma unroll
             for (int d = 0; d < 7; ++d) slot[d] = (x - xa + d) % 7;
-            const LdsStencil<TS, TC> sm{s_mu, slot, base}, sr{s_rho, slot, base};
-            const TC r = sr.at(0, 0, 0);
+            const LdsStencil2<TS, TC> st{s_rm, slot, base};
+            const TC r = (TC)st.at(0, 0, 0).r;
             TC gmx, gmy, gmz, grx, gry, grz;
-            sm.grad(gmx, gmy, gmz);
-            sr.grad(grx, gry, grz);
+            st.grad(grx, gry, grz, gmx, gmy, gmz);
             const TC third = TC(1.0 / 3.0);
             const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
             TC out[Q], u[3];
@@ -702,7 +742,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
                 const bool bounced = (cz(q) < 0 && bb_m) || (cz(q) > 0 && bb_p);
                 TC tref = TC(0);
                 if constexpr (kShifted<TF>)
-                    tref = bounced ? r : (TC)s_rho[slot[3 + cx(q)]][base + cy(q) * MLZ + cz(q)];
+                    tref = bounced ? r : (TC)s_rm[slot[3 + cx(q)]][base + cy(q) * MLZ + cz(q)].r;
                 const TF sv = pop_val<TF, TC>(out[q], q, tref);
                 // streaming store: the populations are read again only next step,
                 // after far more data than the caches hold has passed through

@@ -30,7 +30,7 @@ EXPORTS = [
     "cpmp_set_wall_mu", "cpmp_step", "cpmp_sync", "cpmp_step_count", "cpmp_get_field",
     "cpmp_set_field", "cpmp_reduce", "cpmp_contact_angle", "cpmp_cap_fit", "cpmp_probe",
     "cpmp_laplace_jump",
-    "cpmp_checkpoint_save", "cpmp_checkpoint_load", "cpmp_write_plt", "cpmp_adjoint", "cpmp_adjoint_plan",
+    "cpmp_checkpoint_save", "cpmp_checkpoint_load", "cpmp_write_plt", "cpmp_adjoint", "cpmp_adjoint_plan", "cpmp_neck_radius",
     "cpmp_kernel_times", "cpmp_time_steps", "cpmp_algorithmic_bytes_per_node",
     "cpmp_last_error", "cpmp_version", "cpmp_selftest_collide",
 ]
@@ -118,6 +118,7 @@ def _load():
     L.cpmp_write_plt.argtypes = [p, C.c_char_p, C.c_char_p, i, i]
     L.cpmp_adjoint.argtypes = [p, C.POINTER(AdjDesc), p, p, p, p]
     L.cpmp_adjoint_plan.argtypes = [p, C.POINTER(AdjDesc), C.POINTER(AdjPlan)]
+    L.cpmp_neck_radius.argtypes = [p, C.c_int, C.c_double, C.c_double, p, p]
     L.cpmp_probe.argtypes = [p, i, i, i, i, p]
     L.cpmp_laplace_jump.argtypes = [p, p, p, p]
     L.cpmp_kernel_times.argtypes = [p, i, p, p]
@@ -354,6 +355,12 @@ class Lattice:
         v = np.zeros(1)
         check(lib.cpmp_probe(self.h, fid, int(x), int(y), int(z), _ptr(v)))
         return float(v[0])
+
+    def neck_radius(self, x, rho_g, rho_l):
+        """cpmp_neck_radius: (sqrt(A / pi), A) of the liquid fraction in plane x"""
+        r, a = np.zeros(1), np.zeros(1)
+        check(lib.cpmp_neck_radius(self.h, int(x), rho_g, rho_l, _ptr(r), _ptr(a)))
+        return float(r[0]), float(a[0])
 
     def laplace_jump(self, inside, outside):
         """K^2 [p(rho(inside)) - p(rho(outside))] (cpmp_laplace_jump)"""

@@ -98,6 +98,8 @@ hipError_t launch_velocity(cpmp_ctx *c, double *out3);
 hipError_t launch_plane_y(cpmp_ctx *c, int tier, double *out);
 // per-block (area, max height) partials of the cap fit into part (2 per block)
 hipError_t launch_cap_columns(cpmp_ctx *c, double mid, double *part, unsigned *nblocks);
+// liquid fraction of the fluid nodes of plane x, block partials (nblocks values)
+hipError_t launch_plane_fraction(cpmp_ctx *c, int x, double rg, double rl, double *part, unsigned *nblocks);
 hipError_t launch_force(cpmp_ctx *c, double *out3);
 hipError_t launch_reduce(cpmp_ctx *c, double *host_out /* 12 */);
 hipError_t launch_type(cpmp_ctx *c, double *out);

@@ -1117,6 +1117,21 @@ __global__ void __launch_bounds__(TPB) k_cap_columns(const DevParams P, const TS
     }
 }
 
+// Liquid fraction clamp((rho - rho_g) / (rho_l - rho_g), 0, 1) summed over the fluid
+// nodes of plane x (one contiguous ny*nz block of rho); block partials in part[b]
+template <class TS>
+__global__ void __launch_bounds__(TPB) k_plane_fraction(const DevParams P, const TS *__restrict__ rho, int x, double rg,
+                                                        double rl, double *__restrict__ part) {
+    const long long j = (long long)blockIdx.x * TPB + threadIdx.x;
+    double v = 0;
+    if (j < (long long)P.ny * P.nz && !is_solid(P, (int)(j % P.nz))) {
+        const double phi = ((double)rho[(long long)x * P.ny * P.nz + j] - rg) / (rl - rg);
+        v = phi < 0 ? 0.0 : (phi > 1 ? 1.0 : phi);
+    }
+    const double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // raw copy of every population slot between the device layout (pop_site) and the
 // reference node order
 template <class TF>
@@ -1533,6 +1548,16 @@ hipError_t launch_cap_columns(cpmp_ctx *c, double mid, double *part, unsigned *n
         using TS = typename decltype(fm)::TS;
         hipLaunchKernelGGL(k_cap_columns<TS>, dim3(*nblocks), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, mid,
                            part);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_plane_fraction(cpmp_ctx *c, int x, double rg, double rl, double *part, unsigned *nblocks) {
+    *nblocks = blocks_for((long long)c->p.ny * c->p.nz);
+    return dispatch(c->form, [&](auto fm) {
+        using TS = typename decltype(fm)::TS;
+        hipLaunchKernelGGL(k_plane_fraction<TS>, dim3(*nblocks), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, x, rg,
+                           rl, part);
         return hipGetLastError();
     });
 }

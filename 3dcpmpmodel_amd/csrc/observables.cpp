@@ -114,6 +114,25 @@ int cpmp_cap_fit(cpmp_ctx *c, double rho_g, double rho_l, double *out) {
     return CPMP_OK;
 }
 
+// Radius of the liquid cross-section of plane x (the neck of two coalescing
+// droplets): r = sqrt(A / pi), A the plane's summed liquid fraction.
+int cpmp_neck_radius(cpmp_ctx *c, int x, double rho_g, double rho_l, double *radius, double *area) {
+    if (!c || !c->stream || !radius) return set_error(CPMP_EINVAL, "bad context or output");
+    if (x < 0 || x >= c->p.nx) return set_error(CPMP_EINVAL, "plane x = %d outside the lattice", x);
+    if (!(rho_l > rho_g)) return set_error(CPMP_EINVAL, "rho_l must exceed rho_g");
+    OBS_HIP(hipSetDevice(c->p.device));
+    unsigned nb = 0;
+    OBS_HIP(launch_plane_fraction(c, x, rho_g, rho_l, c->staging, &nb));
+    std::vector<double> part(nb);
+    OBS_HIP(hipMemcpyAsync(part.data(), c->staging, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    OBS_HIP(hipStreamSynchronize(c->stream));
+    double A = 0;
+    for (unsigned b = 0; b < nb; ++b) A += part[b];
+    *radius = std::sqrt(A / 3.14159265358979323846);
+    if (area) *area = A;
+    return CPMP_OK;
+}
+
 // One node's density or chemical potential (FP64), without copying the field.
 int cpmp_probe(cpmp_ctx *c, int field, int x, int y, int z, double *value) {
     if (!c || !c->stream || !value) return set_error(CPMP_EINVAL, "bad context or output");

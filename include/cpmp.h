@@ -207,6 +207,10 @@ int cpmp_cap_fit(cpmp_ctx *ctx, double rho_g, double rho_l, double *out);
 /* Density (CPMP_FIELD_RHO) or chemical potential (CPMP_FIELD_MU) of one node, FP64,
    without copying the field (the reference reads managed Dens / Pote elements). */
 int cpmp_probe(cpmp_ctx *ctx, int field, int x, int y, int z, double *value);
+/* Neck radius of two coalescing droplets (no reference counterpart): in plane x,
+   A = sum over fluid nodes of clamp((rho - rho_g) / (rho_l - rho_g), 0, 1) and
+   radius = sqrt(A / pi).  area may be NULL. */
+int cpmp_neck_radius(cpmp_ctx *ctx, int x, double rho_g, double rho_l, double *radius, double *area);
 /* Laplace pressure jump K^2 [p(rho(in)) - p(rho(out))] between two nodes, with the
    EOS pressure of EosPressure (FluidCuda.cu:1561-1573) for the context's EOS. */
 int cpmp_laplace_jump(cpmp_ctx *ctx, const int *in_xyz, const int *out_xyz, double *dp);

@@ -141,3 +141,35 @@ def test_device_observables_match_host_restatement(cpmp, form):
     with pytest.raises(cpmp.CPMPError):
         g.probe("rho", nx, 0, 0)
     g.close()
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_neck_radius_matches_host_restatement(cpmp, form):
+    """cpmp_neck_radius (device reduction) against the same sum over the downloaded
+    density: a sharp disc along x gives its node count, a smooth column after a few
+    steps the clamped liquid fraction summed on the host"""
+    nx, ny, nz = 8, 48, 40
+    g, l = 0.0319804, 8.25467            # RKE Tr = 0.6 coexistence (CoCurve_RKE_2.txt x Rc)
+    tol = 1e-12 if form == "fp64" else 1e-6
+    y, z = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+    r = np.sqrt((y - 23.5) ** 2 + (z - 19.5) ** 2)
+    disc = r < 12.3
+    lat = cpmp.Lattice(cpmp.make_params(nx, ny, nz, eos="RKE", tr=0.6, nwall=0, form=form))
+    lat.init_from_rho(np.repeat(np.where(disc, l, g)[None], nx, 0).ravel())
+    R, A = lat.neck_radius(3, g, l)
+    assert A == pytest.approx(disc.sum(), rel=tol)
+    assert R == pytest.approx(np.sqrt(disc.sum() / np.pi), rel=tol)
+    lat.close()
+    lat = cpmp.Lattice(cpmp.make_params(nx, ny, nz, eos="RKE", tr=0.6, nwall=0, form=form))
+    prof = g + (l - g) * 0.5 * (1 - np.tanh(2 * (r - 12.3) / 5.0))
+    lat.init_from_rho(np.repeat(prof[None], nx, 0).ravel())
+    lat.step(20)
+    rho = lat.get("rho").reshape(nx, ny, nz)
+    for x in (0, 5):
+        R, A = lat.neck_radius(x, g, l)
+        ref = np.clip((rho[x] - g) / (l - g), 0, 1).sum()
+        print(f"[neck] {form} x={x}: A {A:.6f} host {ref:.6f}, r {R:.4f}", flush=True)
+        assert A == pytest.approx(ref, rel=tol)
+    with pytest.raises(cpmp.CPMPError):
+        lat.neck_radius(nx, g, l)
+    lat.close()

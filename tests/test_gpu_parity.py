@@ -133,3 +133,31 @@ def test_droplet_splash_init_and_parity(cpmp):
     print(f"[parity] splash: max rel err rho after 100 steps = {err:.3e}", flush=True)
     assert err < 1e-10
     g.close()
+
+
+def test_64cubed_carnahan_starling_configuration(cpmp):
+    """the 64^3 configuration in full: Carnahan-Starling at Tr = 0.85, centred sphere
+    of radius 16 with tanh width 2.5, noise 1e-6 (seed 1), kappa 0.01, tau 1, 1000
+    periodic FP64 steps, on the GPU and in the oracle: the density field, the Laplace
+    jump K^2 [p(centre) - p(corner)] and the spurious current max |MV|"""
+    case = dict(dims=(64, 64, 64), eos="CSE", tr=0.85, nwall=0, init="spheres", spheres=[(32, 32, 32, 16)],
+                width=2.5, Ka=0.01, tau=1.0, noise=1e-6, seed=1)
+    o, g = gc.build(case, cpmp)
+    o.step(1000)
+    g.step(1000)
+    rho_o = o.get(orc.RHO)
+    err = gc.rel_err(g.get("rho"), rho_o)
+    mass, bad, ms = o.reduce()
+    s = g.reduce()
+    dp = g.laplace_jump((32, 32, 32), (0, 0, 0))
+    A, B, Tc, Rc = cpmp.set_multiphase("CSE", 0.85)
+    r3 = rho_o.reshape(64, 64, 64)
+    dp_o = g.p.K ** 2 * (cpmp.eos_pressure("CSE", A, B, 0.85 * Tc, r3[32, 32, 32])
+                         - cpmp.eos_pressure("CSE", A, B, 0.85 * Tc, r3[0, 0, 0]))
+    print(f"[c1] rho rel err {err:.2e}; Laplace jump {dp:.6e} (oracle density {dp_o:.6e}); "
+          f"max |MV| {s.max_speed:.6e} (oracle {ms:.6e})", flush=True)
+    assert err < 1e-10
+    assert s.mass == pytest.approx(mass, rel=1e-12)
+    assert dp == pytest.approx(dp_o, rel=1e-9)
+    assert s.max_speed == pytest.approx(ms, rel=1e-8)
+    g.close()

@@ -416,12 +416,15 @@ __device__ __forceinline__ int ring_index(int ly, int lz) {
 // absent terms, so the same values) as each plane enters the ring; the completed
 // density of the workgroup's own tile and chunk is written to rho_out.  This
 // replaces the separate completion pass.
-template <class TS, bool FIX>
+template <class TS, bool FIX, int H>
 __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
                                                   int ntz, int nty, TS *__restrict__ rho_solid, int xc,
                                                   const TS *__restrict__ rbuf, const TS *__restrict__ cbuf,
                                                   TS *__restrict__ rho_out, int nch) {
-    __shared__ TS s_rho[7][MLP];
+    // ring of 2H+1 planes with an in-plane halo of H (2 for the 5-point Laplacian, 3 for
+    // the 7-point one)
+    constexpr int R = 2 * H + 1, LZ = MTZ + 2 * H, LP = (MTY + 2 * H) * LZ;
+    __shared__ TS s_rho[R][LP];
     int t = blockIdx.x;
     if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
     const int tzi = t % ntz;
@@ -434,7 +437,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
     const int y = y0 + ty, z = z0 + tz;
     const bool inr = (y < P.ny) && (z < P.nz);
     const bool active = inr && !is_solid(P, z);
-    constexpr int HE = (MLP + MT - 1) / MT;
+    constexpr int HE = (LP + MT - 1) / MT;
     int hoff[HE], hy[HE], hz[HE];
     bool hown[HE];
     // FIX: in-plane part of the ring index of the (up to three) neighbouring tiles
@@ -447,11 +450,11 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
 #pragma unroll
     for (int h = 0; h < HE; ++h) {
         const int e = threadIdx.x + h * MT;
-        const int ly = e / MLZ, lz = e - ly * MLZ;
-        hy[h] = e < MLP ? wrapc(y0 - MH + ly, P.ny) : 0;
-        hz[h] = e < MLP ? wrapc(z0 - MH + lz, P.nz) : 0;
+        const int ly = e / LZ, lz = e - ly * LZ;
+        hy[h] = e < LP ? wrapc(y0 - H + ly, P.ny) : 0;
+        hz[h] = e < LP ? wrapc(z0 - H + lz, P.nz) : 0;
         hoff[h] = hy[h] * P.nz + hz[h];
-        hown[h] = e < MLP && ly >= MH && ly < MH + MTY && lz >= MH && lz < MH + MTZ;
+        hown[h] = e < LP && ly >= H && ly < H + MTY && lz >= H && lz < H + MTZ;
         if constexpr (FIX) {
             const int gyi = hy[h] / MTY, gzi = hz[h] / MTZ, gy = hy[h] % MTY, gz = hz[h] % MTZ;
             const bool ey0 = gy == 0, ey1 = gy == MTY - 1, ez0 = gz == 0, ez1 = gz == MTZ - 1;
@@ -474,9 +477,9 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
             co[h][3] = (gyi * ntz + gzi) * 2 * AP + (gy + 1) * AZ + gz + 1;
         }
     }
-    // two planes in flight: the halo of plane x+4 and x+5 while plane x computes
-    // (register sets A and B alternate; the loop is unrolled by two so that they
-    // keep fixed registers)
+    // two planes in flight: the halo of planes x+H+1 and x+H+2 while plane x
+    // computes (register sets A and B alternate; the loop is unrolled by two so
+    // that they keep fixed registers)
     TS hA[HE], hB[HE];
     TS fA[FIX ? HE : 1][3], fB[FIX ? HE : 1][3];
     auto fetch_plane = [&](int xg, TS *hr, TS (*fv)[3]) {   // wraps past the chunk (loads unused then)
@@ -485,7 +488,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         const TS *rp = FIX ? rbuf + (long long)xg * plane_ring : nullptr;   // ring entries of plane xg
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) {
+            if ((h + 1) * MT <= LP || threadIdx.x + h * MT < LP) {
                 hr[h] = rho[pb + hoff[h]];
                 if constexpr (FIX) {
 #pragma unroll
@@ -498,7 +501,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         const bool chunk_end = k == 0 || k == xc - 1;   // partial-plane terms too (uniform)
 #pragma unroll
         for (int h = 0; h < HE; ++h)
-            if ((h + 1) * MT <= MLP || threadIdx.x + h * MT < MLP) {
+            if ((h + 1) * MT <= LP || threadIdx.x + h * MT < LP) {
                 TS v = hr[h];
                 if constexpr (FIX) {
                     // k_density_fix's order: ring and partial-plane entry of each
@@ -525,24 +528,24 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
                 s_rho[slot][threadIdx.x + h * MT] = v;
             }
     };
-    for (int k = -MH; k < MH; ++k) { fetch_plane(xa + k, hA, fA); store_plane(k + MH, xa + k, hA, fA); }
-    fetch_plane(xa + MH, hA, fA);
-    fetch_plane(xa + MH + 1, hB, fB);
-    const int base = (ty + MH) * MLZ + tz + MH;
+    for (int k = -H; k < H; ++k) { fetch_plane(xa + k, hA, fA); store_plane(k + H, xa + k, hA, fA); }
+    fetch_plane(xa + H, hA, fA);
+    fetch_plane(xa + H + 1, hB, fB);
+    const int base = (ty + H) * LZ + tz + H;
     const int ym = y < P.ny ? y : 0, zm = z < P.nz ? z : 0;
     auto plane_step = [&](int x, TS *hr, TS (*fv)[3]) {
         __syncthreads();
-        store_plane((x - xa + 2 * MH) % 7, x + MH, hr, fv);   // plane x+3
+        store_plane((x - xa + 2 * H) % R, x + H, hr, fv);   // plane x+H
         __syncthreads();
-        fetch_plane(x + 2 + MH, hr, fv);                      // plane x+5
+        fetch_plane(x + 2 + H, hr, fv);                      // plane x+H+2
         const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
         if (active) {
-            int slot[7];
+            int slot[R];
 #pragma unroll
-            for (int dd = 0; dd < 7; ++dd) slot[dd] = (x - xa + dd) % 7;
-            mu[I] = mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + 3]][base + dy * MLZ + dz]; });
+            for (int dd = 0; dd < R; ++dd) slot[dd] = (x - xa + dd) % R;
+            mu[I] = mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + H]][base + dy * LZ + dz]; });
         } else if (rho_solid && inr) {
-            rho_solid[I] = s_rho[(x - xa + 3) % 7][base];
+            rho_solid[I] = s_rho[(x - xa + H) % R][base];
         }
     };
     int x = xa;
@@ -1304,9 +1307,13 @@ hipError_t launch_mu(cpmp_ctx *c) {
         } else {
             const int xc = c->dp.xc;   // shorter chunks measured slower (0.077 vs 0.071 ms at 256^3)
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + xc - 1) / xc;
-            hipLaunchKernelGGL((k_mu_march<TS, false>), dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0, c->stream, c->dp,
-                               (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc, (const TS *)nullptr,
-                               (const TS *)nullptr, (TS *)nullptr, nch);
+            auto go = [&](auto h_t) {
+                hipLaunchKernelGGL((k_mu_march<TS, false, decltype(h_t)::value>), dim3((unsigned)(ntz * nty * nch)),
+                                   dim3(MT), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc,
+                                   (const TS *)nullptr, (const TS *)nullptr, (TS *)nullptr, nch);
+            };
+            if (c->dp.lap_points == 7) go(std::integral_constant<int, 3>{});
+            else go(std::integral_constant<int, 2>{});
         }
         return hipGetLastError();
     });
@@ -1418,9 +1425,13 @@ static bool fix_deferrable(const cpmp_ctx *c) {
 // the previous collide's tile sums (rho_next) completed into rho, and mu from them
 static hipError_t launch_mu_fix(cpmp_ctx *c) {
     const int xc = c->dp.xc, ntz = c->p.nz / MTZ, nty = c->p.ny / MTY, nch = c->p.nx / xc;
-    hipLaunchKernelGGL((k_mu_march<float, true>), dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0, c->stream, c->dp,
-                       (const float *)c->rho_next, (float *)c->mu, ntz, nty, (float *)nullptr, xc,
-                       (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch);
+    auto go = [&](auto h_t) {
+        hipLaunchKernelGGL((k_mu_march<float, true, decltype(h_t)::value>), dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0,
+                           c->stream, c->dp, (const float *)c->rho_next, (float *)c->mu, ntz, nty, (float *)nullptr, xc,
+                           (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch);
+    };
+    if (c->dp.lap_points == 7) go(std::integral_constant<int, 3>{});
+    else go(std::integral_constant<int, 2>{});
     return hipGetLastError();
 }
 

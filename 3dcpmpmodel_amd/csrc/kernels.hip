@@ -404,6 +404,37 @@ __device__ __forceinline__ int ring_index(int ly, int lz) {
     return 2 * AZ + 2 * (ly - 1) + (lz == AZ - 1 ? 1 : 0);
 }
 
+// Border exchange of the summed density, by receiving tile.  Each tile owns ARING
+// slots per plane for what its neighbours' collides add to its border nodes:
+// row 0 from the tile below (y-1), row MTY-1 from the tile above, column 0 from the
+// z-1 tile, column MTZ-1 from the z+1 tile, and the four corners from the diagonal
+// tiles.  A workgroup thus reads its own border contributions as one contiguous run.
+__device__ __forceinline__ int slot_y0(int tz) { return tz; }
+__device__ __forceinline__ int slot_y1(int tz) { return MTZ + tz; }
+__device__ __forceinline__ int slot_z0(int ty) { return 2 * MTZ + ty; }
+__device__ __forceinline__ int slot_z1(int ty) { return 2 * MTZ + MTY + ty; }
+// corner of the receiving tile fed by the diagonal neighbour at (dy, dz)
+__device__ __forceinline__ int slot_c(int dy, int dz) { return 2 * MTZ + 2 * MTY + (dy > 0 ? 2 : 0) + (dz > 0 ? 1 : 0); }
+// receiving tile (offset dy, dz) and slot of the emitting tile's accumulator border
+// position (ly, lz)
+__device__ __forceinline__ void ring_dest(int ly, int lz, int &dy, int &dz, int &slot) {
+    if (ly == 0) {
+        dy = -1;
+        if (lz == 0) { dz = -1; slot = slot_c(1, 1); }
+        else if (lz == AZ - 1) { dz = 1; slot = slot_c(1, -1); }
+        else { dz = 0; slot = slot_y1(lz - 1); }
+    } else if (ly == AY - 1) {
+        dy = 1;
+        if (lz == 0) { dz = -1; slot = slot_c(-1, 1); }
+        else if (lz == AZ - 1) { dz = 1; slot = slot_c(-1, -1); }
+        else { dz = 0; slot = slot_y0(lz - 1); }
+    } else {
+        dy = 0;
+        if (lz == 0) { dz = -1; slot = slot_z1(ly - 1); }
+        else { dz = 1; slot = slot_z0(ly - 1); }
+    }
+}
+
 // Chemical potential as a march over the same tiles: rho of planes x-3..x+3 with an
 // in-plane halo of 3 sits in an LDS ring, so the 13 (or 19) stencil values of every
 // node come from LDS and each rho value is fetched from memory about once per tile.
@@ -458,22 +489,23 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         if constexpr (FIX) {
             const int gyi = hy[h] / MTY, gzi = hz[h] / MTZ, gy = hy[h] % MTY, gz = hz[h] % MTZ;
             const bool ey0 = gy == 0, ey1 = gy == MTY - 1, ez0 = gz == 0, ez1 = gz == MTZ - 1;
-            auto set = [&](int i, bool on, int dy, int dz, int ly2, int lz2) {
+            const int own_ring = (gyi * ntz + gzi) * ARING;
+            auto set = [&](int i, bool on, int dy, int dz, int ly2, int lz2, int slot) {
                 if (!on) return;
                 const int yy = (gyi + dy + nty) % nty, zz = (gzi + dz + ntz) % ntz;
-                ro[h][i] = (yy * ntz + zz) * ARING + ring_index(ly2, lz2);
+                ro[h][i] = own_ring + slot;
                 co[h][i] = (yy * ntz + zz) * 2 * AP + ly2 * AZ + lz2;
             };
             ro[h][0] = ro[h][1] = ro[h][2] = -1;
             co[h][0] = co[h][1] = co[h][2] = 0;
-            set(0, ey0, -1, 0, AY - 1, gz + 1);
-            set(0, ey1, 1, 0, 0, gz + 1);
-            set(1, ez0, 0, -1, gy + 1, AZ - 1);
-            set(1, ez1, 0, 1, gy + 1, 0);
-            set(2, ey0 && ez0, -1, -1, AY - 1, AZ - 1);
-            set(2, ey0 && ez1, -1, 1, AY - 1, 0);
-            set(2, ey1 && ez0, 1, -1, 0, AZ - 1);
-            set(2, ey1 && ez1, 1, 1, 0, 0);
+            set(0, ey0, -1, 0, AY - 1, gz + 1, slot_y0(gz));
+            set(0, ey1, 1, 0, 0, gz + 1, slot_y1(gz));
+            set(1, ez0, 0, -1, gy + 1, AZ - 1, slot_z0(gy));
+            set(1, ez1, 0, 1, gy + 1, 0, slot_z1(gy));
+            set(2, ey0 && ez0, -1, -1, AY - 1, AZ - 1, slot_c(-1, -1));
+            set(2, ey0 && ez1, -1, 1, AY - 1, 0, slot_c(-1, 1));
+            set(2, ey1 && ez0, 1, -1, 0, AZ - 1, slot_c(1, -1));
+            set(2, ey1 && ez1, 1, 1, 0, 0, slot_c(1, 1));
             co[h][3] = (gyi * ntz + gzi) * 2 * AP + (gy + 1) * AZ + gz + 1;
         }
     }
@@ -685,10 +717,16 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
         }
     };
     // plane xp (xa-1 <= xp <= xb) is complete over this tile and chunk: write it out
+    int rdst = 0;   // receiving tile's slot (in-plane part) of this thread's ring entry
+    if (threadIdx.x < ARING) {
+        int ddy, ddz, sl;
+        ring_dest(rly, rlz, ddy, ddz, sl);
+        rdst = (((tyi + ddy + nty) % nty) * ntz + (tzi + ddz + ntz) % ntz) * ARING + sl;
+    }
     auto flush = [&](int xp, TC own_v, TC ring_v) {
         if (xp >= xa && xp < xb) {
             if (active) rho_next[((long long)xp * P.ny + y) * P.nz + z] = (TS)own_v;
-            if (threadIdx.x < ARING) rbuf[(((long long)xp * nty + tyi) * ntz + tzi) * ARING + threadIdx.x] = (TS)ring_v;
+            if (threadIdx.x < ARING) rbuf[(long long)xp * nty * ntz * ARING + rdst] = (TS)ring_v;
         } else {
             TS *cb = cbuf + ((long long)tile * 2 + (xp < xa ? 0 : 1)) * AP;
             cb[own] = (TS)own_v;
@@ -856,21 +894,21 @@ __global__ void __launch_bounds__(MT) k_density_fix(const DevParams P, TS *__res
         // neighbouring tiles whose border ring holds this node (ring position ly,
         // lz): their ring entry of this plane, and on the chunk's first / last
         // plane their entry of the previous / next chunk's partial plane
-        auto from = [&](bool on, int dy, int dz, int ly, int lz) {
+        auto from = [&](bool on, int dy, int dz, int ly, int lz, int slot) {
             if (!on) return;
             const int i = (dy + 1) * 3 + dz + 1;
-            a += rbuf[(long long)(ch * P.xc + k) * plane_ring + rbase[i] + ring_index(ly, lz)];
+            a += rbuf[(long long)(ch * P.xc + k) * plane_ring + rbase[4] + slot];
             if (lo) a += cbuf[pbase[i] + ly * AZ + lz];
             if (hi) a += cbuf[nbase[i] + ly * AZ + lz];
         };
-        from(ey0, -1, 0, AY - 1, tz + 1);
-        from(ey1, 1, 0, 0, tz + 1);
-        from(ez0, 0, -1, ty + 1, AZ - 1);
-        from(ez1, 0, 1, ty + 1, 0);
-        from(ey0 && ez0, -1, -1, AY - 1, AZ - 1);
-        from(ey0 && ez1, -1, 1, AY - 1, 0);
-        from(ey1 && ez0, 1, -1, 0, AZ - 1);
-        from(ey1 && ez1, 1, 1, 0, 0);
+        from(ey0, -1, 0, AY - 1, tz + 1, slot_y0(tz));
+        from(ey1, 1, 0, 0, tz + 1, slot_y1(tz));
+        from(ez0, 0, -1, ty + 1, AZ - 1, slot_z0(ty));
+        from(ez1, 0, 1, ty + 1, 0, slot_z1(ty));
+        from(ey0 && ez0, -1, -1, AY - 1, AZ - 1, slot_c(-1, -1));
+        from(ey0 && ez1, -1, 1, AY - 1, 0, slot_c(-1, 1));
+        from(ey1 && ez0, 1, -1, 0, AZ - 1, slot_c(1, -1));
+        from(ey1 && ez1, 1, 1, 0, 0, slot_c(1, 1));
         // this tile's own entry of the neighbouring chunks' partial planes
         if (lo) a += cbuf[pbase[4] + (ty + 1) * AZ + tz + 1];
         if (hi) a += cbuf[nbase[4] + (ty + 1) * AZ + tz + 1];

@@ -68,7 +68,8 @@ class AdjDesc(C.Structure):
 
 class AdjPlan(C.Structure):
     _fields_ = [("device_bytes", C.c_int64), ("taped_steps", C.c_int), ("segments", C.c_int),
-                ("sub_length", C.c_int), ("sub_checkpoints", C.c_int), ("recomputed_steps", C.c_int64)]
+                ("sub_length", C.c_int), ("sub_checkpoints", C.c_int), ("recomputed_steps", C.c_int64),
+                ("overlapped", C.c_int)]
 
 
 class CPMPError(RuntimeError):
@@ -331,7 +332,7 @@ class Lattice:
         """cpmp_adjoint.  tape_budget_bytes: trailing steps taped in the first
         forward run instead of recomputed (0: recompute everything, -1: half of
         the free device memory).  memory_limit_bytes: device bytes the call may
-        allocate (0: the free device memory less 1 GiB); segments whose tape does
+        allocate (0: the free device memory less 2 GiB); segments whose tape does
         not fit are recomputed in sub-segments."""
         nxy = self.p.nx * self.p.ny
         t = np.ascontiguousarray(target, np.float64).ravel()

@@ -542,9 +542,11 @@ namespace {
 // steps and recomputed segment by segment with a working tape of S entries.  When
 // S < C a segment is recomputed twice: a first pass stores sub-checkpoints every
 // S steps (and tapes the last sub-segment), then every earlier sub-segment is
-// restored and taped in turn.
+// restored and taped in turn.  With whole segments taped (S = C) and room for a
+// second tape, the reverse sweep overlaps the recompute of the next segment (pipe).
 struct AdjPlan {
     int C = 0, ntaped = 0, n0 = 0, nseg = 0, S = 0, nsubck = 0, tape_len = 1;
+    bool pipe = false;
     size_t per_ck = 0;       // bytes of one checkpoint: populations + rho in the storage form
     size_t per_tape = 0;     // bytes of one tape entry: 10 values of the storage scalar type per node
     size_t nbuf = 0;         // doubles of the cotangent and work buffers
@@ -641,6 +643,29 @@ int make_plan(cpmp_ctx *c, const cpmp_adj_desc *d, AdjPlan &q) {
     q.S = best;
     q.total = plan_total(q, best, &q.nsubck, &q.tape_len);
     q.recomputed = best_rc;
+    // the last segment is taped during the first forward run when the working tape
+    // holds it beside the taped steps: no memory beyond the schedule's, one segment
+    // fewer to recompute
+    if (q.nseg > 0) {
+        const int Lend = q.n0 - (q.nseg - 1) * q.C;
+        if (q.ntaped + Lend <= q.tape_len) {
+            q.ntaped += Lend;
+            q.n0 -= Lend;
+            q.nseg -= 1;
+            q.recomputed -= Lend;
+            q.total = plan_total(q, best, &q.nsubck, &q.tape_len);
+        }
+    }
+    // overlap: at least two blocks to reverse (the taped steps, the segments), whole
+    // segments taped, and a second tape within the limit; CPMP_ADJ_PIPE=0 turns it off
+    const char *pe = std::getenv("CPMP_ADJ_PIPE");
+    const int blocks = q.nseg + (q.ntaped > 0 ? 1 : 0);
+    const size_t piped = q.total + (size_t)q.tape_len * q.per_tape;
+    if (!(pe && std::strcmp(pe, "0") == 0) && q.nseg > 0 && blocks >= 2 && q.S >= std::min(q.C, q.n0) &&
+        piped <= limit) {
+        q.pipe = true;
+        q.total = piped;
+    }
     return CPMP_OK;
 }
 
@@ -658,6 +683,7 @@ extern "C" int cpmp_adjoint_plan(cpmp_ctx *c, const cpmp_adj_desc *d, cpmp_adj_p
     out->sub_length = q.S;
     out->sub_checkpoints = q.nsubck;
     out->recomputed_steps = q.recomputed;
+    out->overlapped = q.pipe ? 1 : 0;
     return CPMP_OK;
 }
 
@@ -690,8 +716,11 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     // the wall fill (N), then u^n (3N), grad mu^n (3N) and grad rho^n (3N) as the
     // forward collide evaluated them
     const size_t sb = c->scal_bytes;
-    char *ckpt = nullptr, *subck = nullptr, *fin = nullptr, *tape = nullptr;
+    char *ckpt = nullptr, *subck = nullptr, *fin = nullptr, *tape = nullptr, *tape2 = nullptr;
     double *buf = nullptr, *tgt = nullptr;
+    // overlapped schedule: the recompute runs on the context's state on stream s2
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_first = nullptr, ev_tape[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     void *fbA, *fbB;
     double *rb, *rbn, *rbl, *Pf, *Wf, *mub, *part, *kpart, *gl, *gh, *acc;
     double acc_h[2] = {0, 0};
@@ -700,6 +729,15 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         if (q.nsubck > 0) AHIP(hipMalloc((void **)&subck, (size_t)q.nsubck * q.per_ck));
         AHIP(hipMalloc((void **)&fin, q.per_ck));
         AHIP(hipMalloc((void **)&tape, (size_t)q.tape_len * q.per_tape));
+        if (q.pipe) {
+            AHIP(hipMalloc((void **)&tape2, (size_t)q.tape_len * q.per_tape));
+            AHIP(hipStreamCreate(&s2));
+            AHIP(hipEventCreateWithFlags(&ev_first, hipEventDisableTiming));
+            for (int i = 0; i < 2; ++i) {
+                AHIP(hipEventCreateWithFlags(&ev_tape[i], hipEventDisableTiming));
+                AHIP(hipEventCreateWithFlags(&ev_rev[i], hipEventDisableTiming));
+            }
+        }
         AHIP(hipMalloc((void **)&buf, q.nbuf * sizeof(double)));
         AHIP(hipMalloc((void **)&tgt, N * sizeof(double)));
         {
@@ -714,35 +752,37 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         AHIP(hipMemcpyAsync(tgt, d->target, N * sizeof(double), hipMemcpyHostToDevice, s));
         AHIP(hipMemsetAsync(buf, 0, q.nbuf * sizeof(double), s));
 
-        // state = populations of the current set + rho, in the storage form
+        // state = populations of the current set + rho, in the storage form.  The
+        // forward work (save, restore, steps) runs on the context's stream: s, or s2
+        // while an overlapped recompute is issued.
         auto save = [&](char *dst) -> hipError_t {
-            hipError_t e = hipMemcpyAsync(dst, c->f[c->cur], fbytes, hipMemcpyDeviceToDevice, s);
-            return e != hipSuccess ? e : hipMemcpyAsync(dst + fbytes, c->rho, rbytes, hipMemcpyDeviceToDevice, s);
+            hipError_t e = hipMemcpyAsync(dst, c->f[c->cur], fbytes, hipMemcpyDeviceToDevice, c->stream);
+            return e != hipSuccess ? e : hipMemcpyAsync(dst + fbytes, c->rho, rbytes, hipMemcpyDeviceToDevice, c->stream);
         };
         auto restore = [&](const char *src) -> hipError_t {
-            hipError_t e = hipMemcpyAsync(c->f[c->cur], src, fbytes, hipMemcpyDeviceToDevice, s);
-            return e != hipSuccess ? e : hipMemcpyAsync(c->rho, src + fbytes, rbytes, hipMemcpyDeviceToDevice, s);
+            hipError_t e = hipMemcpyAsync(c->f[c->cur], src, fbytes, hipMemcpyDeviceToDevice, c->stream);
+            return e != hipSuccess ? e : hipMemcpyAsync(c->rho, src + fbytes, rbytes, hipMemcpyDeviceToDevice, c->stream);
         };
         // one forward step that records its tape entry tr
         auto taped_step = [&](char *tr) -> hipError_t {
             hipError_t e;
             if ((e = launch_wall(c)) != hipSuccess) return e;
-            if ((e = hipMemcpyAsync(tr, c->rho, N * sb, hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+            if ((e = hipMemcpyAsync(tr, c->rho, N * sb, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
             if ((e = launch_mu(c)) != hipSuccess) return e;
             if ((e = launch_collide(c, false, tr + N * sb)) != hipSuccess) return e;
             return launch_macro(c);
         };
-        auto tape_steps = [&](int L) -> hipError_t {
+        auto tape_steps = [&](char *tp, int L) -> hipError_t {
             for (int k = 0; k < L; ++k) {
-                const hipError_t e = taped_step(tape + k * q.per_tape);
+                const hipError_t e = taped_step(tp + k * q.per_tape);
                 if (e != hipSuccess) return e;
             }
             return hipSuccess;
         };
-        // reverse sweep over L taped steps (entries 0..L-1)
-        auto reverse = [&](int L) -> hipError_t {
+        // reverse sweep over L taped steps (entries 0..L-1 of tape tp), on stream s
+        auto reverse = [&](const char *tp, int L) -> hipError_t {
             for (int k = L - 1; k >= 0; --k) {
-                const char *t0 = tape + k * q.per_tape;
+                const char *t0 = tp + k * q.per_tape;
                 auto tape_kernels = [&](auto *tt) {
                     using TT = std::remove_const_t<std::remove_pointer_t<decltype(tt)>>;
                     if (hb == sizeof(double))
@@ -796,7 +836,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
             const int L = std::min(q.C, q.n0 - sg * q.C);
             for (int k = 0; k < L; ++k) AHIP(launch_step(c, false));
         }
-        AHIP(tape_steps(q.ntaped));
+        AHIP(tape_steps(tape, q.ntaped));
         AHIP(save(fin));
         if (sb == sizeof(double))
             hipLaunchKernelGGL(k_adj_J<double>, dim3(nb), dim3(TPB), 0, s, P, (const double *)c->rho, (const double *)tgt, rb, part);
@@ -807,28 +847,60 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         AHIP(hipGetLastError());
 
         // ---- reverse: the taped steps, then the checkpoint segments ----
-        if (q.ntaped > 0) AHIP(reverse(q.ntaped));
-        for (int sg = q.nseg - 1; sg >= 0; --sg) {
-            const char *ck = ckpt + sg * q.per_ck;
-            const int L = std::min(q.C, q.n0 - sg * q.C), S = q.S, nsub = sub_count(L, S);
-            AHIP(restore(ck));
-            if (nsub == 1) {
-                AHIP(tape_steps(L));
-                AHIP(reverse(L));
-                continue;
+        if (q.pipe) {
+            // Block b (the taped steps first if any, then segments nseg-1 .. 0) uses
+            // tape b % 2.  Stream s2 recomputes block b+1 from its checkpoint while s
+            // sweeps block b backwards; s2 refills a tape once the sweep of the block
+            // two before is done, and s starts a block once its tape is recorded.
+            const int first_seg = q.ntaped > 0 ? 1 : 0, nblk = q.nseg + first_seg;
+            char *tb[2] = {tape, tape2};
+            auto seg_of = [&](int b) { return q.nseg - 1 - (b - first_seg); };
+            auto len_of = [&](int b) { return b < first_seg ? q.ntaped : std::min(q.C, q.n0 - seg_of(b) * q.C); };
+            AHIP(hipEventRecord(ev_first, s));     // forward run, final state and J issued
+            AHIP(hipStreamWaitEvent(s2, ev_first, 0));
+            auto recompute_block = [&](int b) -> hipError_t {
+                const int i = b & 1;
+                hipError_t e = hipSuccess;
+                if (b >= 2) e = hipStreamWaitEvent(s2, ev_rev[i], 0);
+                c->stream = s2;
+                if (e == hipSuccess) e = restore(ckpt + seg_of(b) * q.per_ck);
+                if (e == hipSuccess) e = tape_steps(tb[i], len_of(b));
+                c->stream = s;
+                if (e == hipSuccess) e = hipEventRecord(ev_tape[i], s2);
+                return e;
+            };
+            if (first_seg == 0) AHIP(recompute_block(0));
+            for (int b = 0; b < nblk; ++b) {
+                if (b + 1 < nblk) AHIP(recompute_block(b + 1));
+                const int i = b & 1;
+                if (b >= first_seg) AHIP(hipStreamWaitEvent(s, ev_tape[i], 0));
+                AHIP(reverse(tb[i], len_of(b)));
+                AHIP(hipEventRecord(ev_rev[i], s));
             }
-            // first pass: sub-checkpoints at S, 2S, .. (nsub-2)S, last sub-segment taped
-            for (int j = 1; j < nsub; ++j) {
-                for (int k = 0; k < S; ++k) AHIP(launch_step(c, false));
-                if (j <= nsub - 2) AHIP(save(subck + (j - 1) * q.per_ck));
-            }
-            const int Llast = L - (nsub - 1) * S;
-            AHIP(tape_steps(Llast));
-            AHIP(reverse(Llast));
-            for (int j = nsub - 2; j >= 0; --j) {
-                AHIP(restore(j == 0 ? ck : subck + (j - 1) * q.per_ck));
-                AHIP(tape_steps(S));
-                AHIP(reverse(S));
+        } else {
+            if (q.ntaped > 0) AHIP(reverse(tape, q.ntaped));
+            for (int sg = q.nseg - 1; sg >= 0; --sg) {
+                const char *ck = ckpt + sg * q.per_ck;
+                const int L = std::min(q.C, q.n0 - sg * q.C), S = q.S, nsub = sub_count(L, S);
+                AHIP(restore(ck));
+                if (nsub == 1) {
+                    AHIP(tape_steps(tape, L));
+                    AHIP(reverse(tape, L));
+                    continue;
+                }
+                // first pass: sub-checkpoints at S, 2S, .. (nsub-2)S, last sub-segment taped
+                for (int j = 1; j < nsub; ++j) {
+                    for (int k = 0; k < S; ++k) AHIP(launch_step(c, false));
+                    if (j <= nsub - 2) AHIP(save(subck + (j - 1) * q.per_ck));
+                }
+                const int Llast = L - (nsub - 1) * S;
+                AHIP(tape_steps(tape, Llast));
+                AHIP(reverse(tape, Llast));
+                for (int j = nsub - 2; j >= 0; --j) {
+                    AHIP(restore(j == 0 ? ck : subck + (j - 1) * q.per_ck));
+                    AHIP(tape_steps(tape, S));
+                    AHIP(reverse(tape, S));
+                }
             }
         }
     }
@@ -849,6 +921,14 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
     c->step = step0 + d->nsteps;
     c->mv_valid = false;
 done:
+    c->stream = s;
+    if (s2) {
+        (void)hipStreamSynchronize(s2);
+        (void)hipStreamDestroy(s2);
+    }
+    for (hipEvent_t ev : {ev_first, ev_tape[0], ev_tape[1], ev_rev[0], ev_rev[1]})
+        if (ev) (void)hipEventDestroy(ev);
+    if (tape2) (void)hipFree(tape2);
     if (ckpt) (void)hipFree(ckpt);
     if (subck) (void)hipFree(subck);
     if (fin) (void)hipFree(fin);

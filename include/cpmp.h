@@ -115,10 +115,11 @@ typedef struct {
        trailing steps recorded during the first forward run; those steps are not
        recomputed.  The steps before them are covered by checkpoints every
        checkpoint_every steps and recomputed segment by segment.  0: checkpoints and
-       recompute for every step; < 0: half of the free device memory. */
+       recompute for every step but those of the last segment, which the working tape
+       holds during the first run; < 0: half of the free device memory. */
     int64_t tape_budget_bytes;
     /* Limit on the device bytes the call allocates (0: the free device memory less
-       1 GiB).  A segment whose tape does not fit is recomputed in sub-segments:
+       2 GiB).  A segment whose tape does not fit is recomputed in sub-segments:
        a first pass stores sub-checkpoints, then each sub-segment is taped in turn
        (each of its steps is recomputed twice).  The result does not depend on the
        schedule. */
@@ -133,6 +134,10 @@ typedef struct {
     int sub_length;            /* steps per taped sub-segment (= segment length when it fits) */
     int sub_checkpoints;       /* extra states stored per segment for the sub-segments */
     int64_t recomputed_steps;  /* forward steps run again during the reverse sweep */
+    /* 1: each segment is recomputed into a second tape on a second stream while the
+       reverse sweep of the segment after it runs (two tapes in device_bytes); 0: one
+       tape, recompute and reverse sweep in turn.  The result does not depend on it. */
+    int overlapped;
 } cpmp_adj_plan;
 
 typedef struct cpmp_ctx cpmp_ctx;

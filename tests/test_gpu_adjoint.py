@@ -233,6 +233,32 @@ def test_adjoint_tape_budget_schedules_agree(cpmp):
         assert r[3] == res[0][3]
 
 
+@pytest.mark.parametrize("form,dims", [("fp64", (16, 12, 20)), ("fp32", (32, 16, 32))])
+def test_adjoint_overlapped_schedule_matches_serial(cpmp, monkeypatch, form, dims):
+    """segments recomputed on a second stream while the reverse sweep of the
+    segment after them runs (two tapes), with and without trailing taped steps:
+    the same J and gradients, bit for bit, as recompute and sweep in turn"""
+    n = dims[0] * dims[1] * dims[2]
+    per_step = 10 * n * (8 if form == "fp64" else 4)
+    for budget in (0, 13 * per_step):
+        res = []
+        for pipe in ("1", "0"):
+            monkeypatch.setenv("CPMP_ADJ_PIPE", pipe)
+            g, target = _small_lattice(cpmp, dims, form)
+            plan = g.adjoint_plan(40, 6, tape_budget_bytes=budget)
+            assert plan["overlapped"] == (1 if pipe == "1" else 0)
+            res.append(g.adjoint(40, target, checkpoint_every=6, tape_budget_bytes=budget))
+            assert g.step_count == 40
+            rho_end = g.get("rho")
+            g.close()
+            res[-1] = res[-1] + (rho_end,)
+        print(f"[adjoint-overlap] {form} budget {budget}: J {res[0][0]:.12e}", flush=True)
+        a, b = res
+        assert a[0] == b[0] and a[3] == b[3]
+        for i in (1, 2, 4):
+            assert np.array_equal(a[i], b[i])
+
+
 def _small_lattice(cpmp, dims, form="fp64", **extra):
     kw, rho0, mu_lo, mu_hi, target = small_case(cpmp, *dims)
     kw.update(extra)
@@ -266,7 +292,9 @@ def test_adjoint_memory_limit_sub_segments_agree(cpmp):
         g.close()
     print(f"[adjoint-plan] {plans}", flush=True)
     # total(S) = base + S * per_tape + max(ceil(40 / S) - 2, 0) * (160 B per node)
-    assert (plans[0]["sub_length"], plans[0]["sub_checkpoints"], plans[0]["recomputed_steps"]) == (40, 0, 40)
+    # without a limit the one segment fits the working tape: taped in the first run
+    assert (plans[0]["sub_length"], plans[0]["sub_checkpoints"], plans[0]["recomputed_steps"]) == (40, 0, 0)
+    assert (plans[0]["taped_steps"], plans[0]["segments"]) == (40, 0)
     assert (plans[1]["sub_length"], plans[1]["sub_checkpoints"], plans[1]["recomputed_steps"]) == (14, 1, 68)
     assert (plans[2]["sub_length"], plans[2]["sub_checkpoints"], plans[2]["recomputed_steps"]) == (10, 2, 70)
     assert plans[1]["device_bytes"] == base + 1280 * n and plans[2]["device_bytes"] == base + 1120 * n

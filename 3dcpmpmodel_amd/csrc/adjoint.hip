@@ -209,13 +209,11 @@ constexpr int kAdjSlots = (kAdjEA + TPB - 1) / TPB;
 static_assert(kAdjTY * kAdjTZ == TPB, "one thread per tile column");
 
 template <bool SUB>
-__global__ void __launch_bounds__(TPB) k_adj_stencil_march(const DevParams P, const double *__restrict__ A,
-                                                           const double *__restrict__ base, double *__restrict__ out,
-                                                           int ntz, int nty, int xc) {
-    __shared__ double sf[3][kAdjEA];   // x, y, z components of one plane
+__device__ __forceinline__ void adj_stencil_march(const DevParams &P, const double *__restrict__ A,
+                                                  const double *__restrict__ base, double *__restrict__ out, int ntz,
+                                                  int nty, int xc, int b, double (*sf)[kAdjEA]) {
     const long long N = P.N, plane = (long long)P.ny * P.nz;
     const int tid = threadIdx.x, tz = tid % kAdjTZ, ty = tid / kAdjTZ;
-    int b = blockIdx.x;
     const int zt = b % ntz;
     b /= ntz;
     const int yt = b % nty, ch = b / nty;
@@ -283,6 +281,20 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil_march(const DevParams P, co
         for (int k = 0; k < 6; ++k) acc[k] = acc[k + 1];
         acc[6] = 0;
     }
+}
+
+// Both transposed stencils of one reverse step in one launch: workgroups
+// 0 .. nb-1 take P_f -> mu cotangent, nb .. 2 nb-1 take W_f -> rho cotangent
+// (independent outputs; one launch fills the device twice as well on small lattices).
+__global__ void __launch_bounds__(TPB) k_adj_stencil_march(const DevParams P, const double *__restrict__ Pf,
+                                                           const double *__restrict__ Wf,
+                                                           const double *__restrict__ rb_loc, double *__restrict__ mub,
+                                                           double *__restrict__ rb_new, int ntz, int nty, int xc,
+                                                           int nb) {
+    __shared__ double sf[3][kAdjEA];   // x, y, z components of one plane
+    const int b = blockIdx.x;
+    if (b < nb) adj_stencil_march<false>(P, Pf, nullptr, mub, ntz, nty, xc, b, sf);
+    else adj_stencil_march<true>(P, Wf, rb_loc, rb_new, ntz, nty, xc, b - nb, sf);
 }
 
 // transposed chemical potential: Laplacian of the fluid mu cotangent, EOS slope,
@@ -793,12 +805,11 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
                         hipLaunchKernelGGL((k_adj_collide<TT, float>), node_grid(P), dim3(TPB), 0, s, P,
                                            (const float *)fbA, (const double *)rb, tt, tt + N, tt + 4 * N, (float *)fbB,
                                            rbl, Pf, Wf);
-                    if (march) {
-                        hipLaunchKernelGGL(k_adj_stencil_march<false>, dim3(march_blocks), dim3(TPB), 0, s, P,
-                                           (const double *)Pf, (const double *)nullptr, mub, ntz_a, nty_a, xc_a);
-                        hipLaunchKernelGGL(k_adj_stencil_march<true>, dim3(march_blocks), dim3(TPB), 0, s, P,
-                                           (const double *)Wf, (const double *)rbl, rbn, ntz_a, nty_a, xc_a);
-                    } else
+                    if (march)
+                        hipLaunchKernelGGL(k_adj_stencil_march, dim3(2 * march_blocks), dim3(TPB), 0, s, P,
+                                           (const double *)Pf, (const double *)Wf, (const double *)rbl, mub, rbn, ntz_a,
+                                           nty_a, xc_a, (int)march_blocks);
+                    else
                         hipLaunchKernelGGL(k_adj_stencil, node_grid(P), dim3(TPB), 0, s, P, (const double *)Pf,
                                            (const double *)Wf, (const double *)rbl, mub, rbn);
                     if (!march_mu)

@@ -491,10 +491,13 @@ __global__ void __launch_bounds__(kWallTile * kWallTile) k_adj_wall_tile(const D
     auto at = [&](int xx, int yy, int k) { return ((long long)wrapc(xx, P.nx) * P.ny + wrapc(yy, P.ny)) * P.nz + k; };
     const long long col = (long long)x * P.ny + y;
     if (own) {   // wall-field gradient: solid mu = mu_w of the column
-        const double *m = mub + col * P.nz;
+        const double *m = mub + col * P.nz + (top ? P.nz - P.hi : 0);
         double g = 0;
-        if (top) for (int k = P.nz - P.hi; k < P.nz; ++k) g += m[k];
-        else for (int k = 0; k < P.lo; ++k) g += m[k];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {   // loads issued together (layers past depth read layer 0, add 0)
+            const double t = m[k < depth ? k : 0];
+            g += k < depth ? t : 0.0;
+        }
         (top ? ghi : glo)[col] += g;
     }
     if (!P.wall_scheme) {
@@ -508,13 +511,22 @@ __global__ void __launch_bounds__(kWallTile * kWallTile) k_adj_wall_tile(const D
         rb[at(x, y, kf)] = acc;
         return;
     }
-    // layer d on the patch [R - d, E - R + d)^2
-    for (int d = 0; d <= depth; ++d)
-        for (int i = threadIdx.x; i < E * E; i += NT) {
-            const int a = i % E, b = i / E;
-            if (a < R - d || a >= E - R + d || b < R - d || b >= E - R + d) continue;
-            L[d][i] = rb[at(x0 - R + a, y0 - R + b, kf - d * step)];
-        }
+    // layer d on the patch [R - d, E - R + d)^2 (0 elsewhere); every load of the
+    // thread in flight at once
+    constexpr int LA = (R + 1) * E * E, PER = (LA + NT - 1) / NT;
+    double v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = (int)threadIdx.x + j * NT;
+        const int d = i / (E * E), e = i - d * (E * E), a = e % E, b = e / E;
+        const bool use = i < LA && d <= depth && a >= R - d && a < E - R + d && b >= R - d && b < E - R + d;
+        v[j] = use ? rb[at(x0 - R + a, y0 - R + b, kf - d * step)] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = (int)threadIdx.x + j * NT;
+        if (i < LA) (&L[0][0])[i] = v[j];
+    }
     __syncthreads();
     for (int d = depth - 1; d >= 0; --d) {
         const double *c = L[d + 1];
@@ -779,8 +791,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         auto taped_step = [&](char *tr) -> hipError_t {
             hipError_t e;
             if ((e = launch_wall(c)) != hipSuccess) return e;
-            if ((e = hipMemcpyAsync(tr, c->rho, N * sb, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-            if ((e = launch_mu(c)) != hipSuccess) return e;
+            if ((e = launch_mu(c, tr)) != hipSuccess) return e;   // also copies rho (after the wall fill) to tr
             if ((e = launch_collide(c, false, tr + N * sb)) != hipSuccess) return e;
             return launch_macro(c);
         };

@@ -69,7 +69,8 @@ int set_error(int code, const char *fmt, ...);
 
 // launchers (kernels.hip); all enqueue on ctx->stream
 hipError_t launch_wall(cpmp_ctx *c);
-hipError_t launch_mu(cpmp_ctx *c);
+// rho_copy (optional): receives a copy of rho as the kernel reads it (the adjoint's tape)
+hipError_t launch_mu(cpmp_ctx *c, void *rho_copy = nullptr);
 bool density_fused(const cpmp_ctx *c);
 bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *chunk_elems);
 int march_chunk(int nx, int ny, int nz);

@@ -579,6 +579,8 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         } else if (rho_solid && inr) {
             rho_solid[I] = s_rho[(x - xa + H) % R][base];
         }
+        // without FIX, rho_out (if given) receives a copy of rho (the adjoint's tape)
+        if (!FIX && rho_out && inr) rho_out[I] = s_rho[(x - xa + H) % R][base];
     };
     int x = xa;
     for (; x + 1 < xb; x += 2) {
@@ -1335,11 +1337,16 @@ template <class TS> static const TS *ref_buffer(const cpmp_ctx *c) {
     return c->form == CPMP_FORM_FP16 ? (const TS *)c->rho_next : nullptr;
 }
 
-hipError_t launch_mu(cpmp_ctx *c) {
+hipError_t launch_mu(cpmp_ctx *c, void *rho_copy) {
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
         TS *solid = (TS *)c->rho_next;   // double-buffered density: solid values follow the step
         if (mu_variant() == 1) {
+            if (rho_copy) {
+                const hipError_t e = hipMemcpyAsync(rho_copy, c->rho, (size_t)c->N * sizeof(TS), hipMemcpyDeviceToDevice,
+                                                    c->stream);
+                if (e != hipSuccess) return e;
+            }
             hipLaunchKernelGGL(k_mu<TS>, node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho,
                                (TS *)c->mu, solid);
         } else {
@@ -1348,7 +1355,7 @@ hipError_t launch_mu(cpmp_ctx *c) {
             auto go = [&](auto h_t) {
                 hipLaunchKernelGGL((k_mu_march<TS, false, decltype(h_t)::value>), dim3((unsigned)(ntz * nty * nch)),
                                    dim3(MT), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc,
-                                   (const TS *)nullptr, (const TS *)nullptr, (TS *)nullptr, nch);
+                                   (const TS *)nullptr, (const TS *)nullptr, (TS *)rho_copy, nch);
             };
             if (c->dp.lap_points == 7) go(std::integral_constant<int, 3>{});
             else go(std::integral_constant<int, 2>{});

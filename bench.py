@@ -364,7 +364,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
-        cpu = cpu_baseline(n, 3)
+        cpu = cpu_baseline(n, 18)   # about 10 s of CPU work at 256^3 on 16 threads
         log(f"[bench] cpu: {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
     out = {
         "metric": "MLUPS 256^3 D3Q19 FP32 vs FP64 at <=0.5deg contact-angle error; adjoint/fwd cost",

@@ -53,6 +53,14 @@ hipError_t sync_reference_density(cpmp_ctx *c) {
 int pop_size(int form) { return form == CPMP_FORM_FP64 ? 8 : (form == CPMP_FORM_FP32 ? 4 : 2); }
 int scal_size(int form) { return form == CPMP_FORM_FP64 ? 8 : 4; }
 
+// interface runs of the chemical-potential march (FP32 storage forms): 64 * xc
+// slots per wave, MT / 64 waves per workgroup
+size_t band_slots(const cpmp_ctx *c) {
+    const size_t xc = (size_t)c->dp.xc;
+    const size_t blocks = (size_t)((c->p.nz + kTileZ - 1) / kTileZ) * ((c->p.ny + kTileY - 1) / kTileY) * ((c->p.nx + xc - 1) / xc);
+    return blocks * kTileY * kTileZ * xc;
+}
+
 void fill_devparams(cpmp_ctx *c) {
     const cpmp_params &p = c->p;
     DevParams &d = c->dp;
@@ -68,6 +76,18 @@ void fill_devparams(cpmp_ctx *c) {
         if (cpmp::unit_rate_row(q) && p.Ts[q] != 1.0) d.unit_rates = 0;
     d.rho0 = p.rho0;
     d.xc = march_chunk(p.nx, p.ny, p.nz);
+    // interface band of the FP64 EOS chemical potential (the last error is kept)
+    d.band_lo = 1.0f;
+    d.band_hi = 0.0f;
+    {
+        const std::string saved = g_err;
+        double g = 0, l = 0;
+        if (cpmp_coexistence(p.eos, p.A, p.B, p.T, &g, &l) == CPMP_OK) {
+            d.band_lo = (float)(g + 0.02 * (l - g));
+            d.band_hi = (float)(l - 0.02 * (l - g));
+        }
+        g_err = saved;
+    }
 }
 
 int check_params(const cpmp_params *p) {
@@ -275,6 +295,8 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
         if (hipMalloc(&c->rho_next, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "previous rho");
         c->fuse_bytes = (size_t)c->N * c->scal_bytes;
     }
+    if (c->form != CPMP_FORM_FP64 && hipMalloc((void **)&c->band, band_slots(c) * sizeof(uint4)) != hipSuccess)
+        return fail(CPMP_ENOMEM, "interface runs");
     if (hipMalloc((void **)&c->mv, 3 * c->N * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "MV");
     if (hipMalloc((void **)&c->mu_w, 2 * c->NXY * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "wall mu");
     if (hipMalloc((void **)&c->red, reduce_scratch_doubles() * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "reduce");
@@ -294,7 +316,8 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
 int cpmp_destroy(cpmp_ctx *c) {
     if (!c) return CPMP_OK;
     if (c->stream) { hipSetDevice(c->p.device); hipStreamSynchronize(c->stream); }
-    for (void *ptr : {c->f[0], c->f[1], c->rho, c->rho_next, c->rbuf, c->cbuf, c->mu, (void *)c->mv, (void *)c->mu_w, (void *)c->red, (void *)c->staging})
+    for (void *ptr : {c->f[0], c->f[1], c->rho, c->rho_next, c->rbuf, c->cbuf, c->mu, (void *)c->band, (void *)c->mv,
+                      (void *)c->mu_w, (void *)c->red, (void *)c->staging})
         if (ptr) hipFree(ptr);
     if (c->timers) {
         for (auto &e : c->timers->ev) if (e) hipEventDestroy(e);
@@ -314,7 +337,8 @@ int cpmp_get_params(cpmp_ctx *c, cpmp_params *out) {
 int cpmp_memory(cpmp_ctx *c, int64_t *bytes) {
     if (!valid_ctx(c) || !bytes) return set_error(CPMP_EINVAL, "bad context or output");
     *bytes = (int64_t)(2 * (size_t)c->N * 19 * c->pop_bytes + 2 * c->N * c->scal_bytes + 4 * c->N * sizeof(double)
-                       + 2 * c->NXY * sizeof(double) + reduce_scratch_doubles() * sizeof(double) + c->fuse_bytes);
+                       + 2 * c->NXY * sizeof(double) + reduce_scratch_doubles() * sizeof(double) + c->fuse_bytes
+                       + (c->band ? band_slots(c) * sizeof(uint4) : 0));
     return CPMP_OK;
 }
 

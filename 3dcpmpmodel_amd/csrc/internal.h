@@ -26,6 +26,11 @@ struct DevParams {
     int tiled, nty, ntz;        // tile-contiguous population sites (pop_site), tile counts in y, z
     int xc;                     // x planes per workgroup of the march kernels
     int unit_rates;             // the rates of rows 0, 1, 3, 5, 7, 10, 12, 16-18 are all 1 (reference set)
+    // FP32 storage forms: fluid nodes with a density strictly inside (band_lo, band_hi)
+    // -- the interface: between the Maxwell gas and liquid densities, less 2 % of their
+    // gap at each end -- get the EOS chemical potential in FP64; empty (lo > hi) without
+    // a two-phase region
+    float band_lo, band_hi;
 };
 
 enum Kern { KW = CPMP_K_WALL, KM = CPMP_K_MU, KC = CPMP_K_COLLIDE, KR = CPMP_K_MACRO };
@@ -53,6 +58,9 @@ struct cpmp_ctx {
     // FP32 storage forms on exactly tiled lattices: density summed by the collide
     // kernel into rho_next, tile rings and chunk partial planes for the fix-up pass
     void *rho_next = nullptr, *rbuf = nullptr, *cbuf = nullptr;
+    // FP32 storage forms: per wave of the chemical-potential march, a run of 64 * xc
+    // slots (node, density, Laplacian term) for the interface nodes it finishes in FP64
+    uint4 *band = nullptr;
     size_t fuse_bytes = 0;
     double *mv = nullptr;       // 3*N, MV of the last step
     double *mu_w = nullptr;     // 2*NXY: bottom then top

@@ -175,9 +175,18 @@ __device__ __forceinline__ float mu_eos_fast(int eos, int form, float A, float B
     return T * (ln_hw(d) + (8.0f * bd - 9.0f * bd * bd + 3.0f * bd * bd * bd) * rom * rom * rom) - 2.0f * A * d;
 }
 
-// mu at one fluid node; get(dx, dy, dz) returns rho at the offset node
-template <class TS, class Get>
-__device__ __forceinline__ TS mu_node(const DevParams &P, Get get) {
+// FP32 mu of an interface node from its FP32 Laplacian term Ka * lap: the EOS part
+// in FP64, rounded once
+__device__ __forceinline__ float mu_band(const DevParams &P, float d, float kal) {
+    return (float)(P.K2s * mu_eos_t<double>(P.eos, P.mu_form, P.A, P.B, P.T, clamp_rho((double)d)) - (double)kal);
+}
+__device__ __forceinline__ bool in_band(const DevParams &P, float d) { return d > P.band_lo && d < P.band_hi; }
+
+// mu at one fluid node; get(dx, dy, dz) returns rho at the offset node.  FP32:
+// interface nodes (in_band) get the EOS part in FP64 if EOS64, else the FP32 value;
+// kal_out (if given) receives the FP32 Laplacian term.
+template <class TS, bool EOS64 = true, class Get>
+__device__ __forceinline__ TS mu_node(const DevParams &P, Get get, float *kal_out = nullptr) {
     if constexpr (sizeof(TS) == 8) {
         const double d = get(0, 0, 0);
         double kal;
@@ -213,9 +222,32 @@ __device__ __forceinline__ TS mu_node(const DevParams &P, Get get) {
             };
             lap = (ax(1, 0, 0) + ax(0, 1, 0) + ax(0, 0, 1)) * (1.0f / 12.0f);
         }
+        const float kal = (float)P.Ka * lap;
+        if (kal_out) *kal_out = kal;
+        if (EOS64 && in_band(P, d)) return mu_band(P, d, kal);
         const float dc = d > 9.7f ? 9.7f + (d - 9.7f) * rcp_hw(1.0f + (d - 9.7f) * 1.5f) : d;   // clamp_rho
         const float m = mu_eos_fast(P.eos, P.mu_form, (float)P.A, (float)P.B, (float)P.T, dc);
-        return (float)P.K2s * m - (float)P.Ka * lap;
+        return (float)P.K2s * m - kal;
+    }
+}
+
+// The chemical-potential march finishes its interface nodes at the end: each wave
+// appends (node, density, Laplacian term) to its run as it goes (lanes in lane
+// order; count is the wave's running total, the same in every lane), then sets
+// their mu with the FP64 EOS part, 64 at a time.
+__device__ __forceinline__ void band_append(bool in, unsigned idx, float d, float kal, uint4 *run, unsigned &count) {
+    const unsigned long long m = __ballot(in);
+    const int lane = (int)__lane_id();
+    if (in) run[count + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] =
+        make_uint4(idx, __float_as_uint(d), __float_as_uint(kal), 0u);
+    count += (unsigned)__popcll(m);
+}
+__device__ __forceinline__ void band_finish(const DevParams &P, const uint4 *run, unsigned count, float *mu) {
+    if (count == 0) return;   // uniform over the wave
+    __threadfence_block();    // entries written by other lanes are visible
+    for (unsigned i = __lane_id(); i < count; i += 64) {
+        const uint4 e = run[i];
+        mu[e.x] = mu_band(P, __uint_as_float(e.y), __uint_as_float(e.z));
     }
 }
 
@@ -451,11 +483,14 @@ template <class TS, bool FIX, int H>
 __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
                                                   int ntz, int nty, TS *__restrict__ rho_solid, int xc,
                                                   const TS *__restrict__ rbuf, const TS *__restrict__ cbuf,
-                                                  TS *__restrict__ rho_out, int nch) {
+                                                  TS *__restrict__ rho_out, int nch, uint4 *__restrict__ band) {
     // ring of 2H+1 planes with an in-plane halo of H (2 for the 5-point Laplacian, 3 for
     // the 7-point one)
     constexpr int R = 2 * H + 1, LZ = MTZ + 2 * H, LP = (MTY + 2 * H) * LZ;
     __shared__ TS s_rho[R][LP];
+    // FP32: this wave's run of interface nodes and their count
+    uint4 *band_run = band ? band + ((size_t)blockIdx.x * (MT / 64) + threadIdx.x / 64) * 64 * xc : nullptr;
+    unsigned band_count = 0;
     int t = blockIdx.x;
     if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
     const int tzi = t % ntz;
@@ -571,14 +606,21 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         __syncthreads();
         fetch_plane(x + 2 + H, hr, fv);                      // plane x+H+2
         const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
+        float kal = 0.0f;
         if (active) {
             int slot[R];
 #pragma unroll
             for (int dd = 0; dd < R; ++dd) slot[dd] = (x - xa + dd) % R;
-            mu[I] = mu_node<TS>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + H]][base + dy * LZ + dz]; });
+            mu[I] = mu_node<TS, false>(P, [&](int dx, int dy, int dz) { return s_rho[slot[dx + H]][base + dy * LZ + dz]; },
+                                       &kal);
         } else if (rho_solid && inr) {
             rho_solid[I] = s_rho[(x - xa + H) % R][base];
         }
+        if constexpr (sizeof(TS) == 4)
+            if (band) {
+                const float dcen = s_rho[(x - xa + H) % R][base];
+                band_append(active && in_band(P, dcen), (unsigned)I, dcen, kal, band_run, band_count);
+            }
         // without FIX, rho_out (if given) receives a copy of rho (the adjoint's tape)
         if (!FIX && rho_out && inr) rho_out[I] = s_rho[(x - xa + H) % R][base];
     };
@@ -588,6 +630,8 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         plane_step(x + 1, hB, fB);
     }
     if (x < xb) plane_step(x, hA, fA);
+    if constexpr (sizeof(TS) == 4)
+        if (band) band_finish(P, band_run, band_count, (float *)mu);
 }
 
 // With RS the kernel also sums the next step's density: every post-collision
@@ -1355,7 +1399,7 @@ hipError_t launch_mu(cpmp_ctx *c, void *rho_copy) {
             auto go = [&](auto h_t) {
                 hipLaunchKernelGGL((k_mu_march<TS, false, decltype(h_t)::value>), dim3((unsigned)(ntz * nty * nch)),
                                    dim3(MT), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc,
-                                   (const TS *)nullptr, (const TS *)nullptr, (TS *)rho_copy, nch);
+                                   (const TS *)nullptr, (const TS *)nullptr, (TS *)rho_copy, nch, c->band);
             };
             if (c->dp.lap_points == 7) go(std::integral_constant<int, 3>{});
             else go(std::integral_constant<int, 2>{});
@@ -1473,7 +1517,7 @@ static hipError_t launch_mu_fix(cpmp_ctx *c) {
     auto go = [&](auto h_t) {
         hipLaunchKernelGGL((k_mu_march<float, true, decltype(h_t)::value>), dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0,
                            c->stream, c->dp, (const float *)c->rho_next, (float *)c->mu, ntz, nty, (float *)nullptr, xc,
-                           (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch);
+                           (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch, c->band);
     };
     if (c->dp.lap_points == 7) go(std::integral_constant<int, 3>{});
     else go(std::integral_constant<int, 2>{});

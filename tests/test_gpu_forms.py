@@ -27,6 +27,57 @@ def test_reduced_precision_forms_track_fp64(cpmp, form, tol):
     assert abs(ml - m64) / m64 < tol
 
 
+def _ulp32(x):
+    a = np.abs(x).astype(np.float32)
+    return (np.nextafter(a, np.float32(np.inf)) - a).astype(np.float64)
+
+
+@pytest.mark.parametrize("form", ["fp32", "fp16"])
+def test_interface_chemical_potential_in_fp64(cpmp, form):
+    """The FP32 storage forms evaluate the EOS part of mu in FP64 at interface nodes
+    (density inside the Maxwell gas-liquid gap less 2 % at each end) and round once:
+    there mu matches the FP64 form's mu of the same (FP32) density field to 1.5 FP32
+    ulp, both for the chemical-potential march that starts a run and for the one
+    that completes the previous step's summed density (the second step of a run on a
+    periodic tiled lattice)."""
+    dims = (64, 32, 64)
+    eos, tr = "RKE", 0.6
+    A, B, Tc, Rc = cpmp.set_multiphase(eos, tr)
+    g0, l0 = cpmp.coexistence(eos, A, B, tr * Tc)
+    lo, hi = np.float32(g0 + 0.02 * (l0 - g0)), np.float32(l0 - 0.02 * (l0 - g0))
+
+    def lat(f):
+        return cpmp.Lattice(cpmp.make_params(*dims, eos=eos, tr=tr, nwall=0, form=f, rho0=g0))
+
+    def start(g):
+        g.init_spheres([(32, 16, 32, 14)], 5.0, g0, l0, noise_amp=1e-3, seed=5)
+
+    a = lat(form)
+    start(a)
+    r0 = a.get("rho")
+    a.step(1)
+    r1, mu_first = a.get("rho"), a.get("mu")
+    a.close()
+    b = lat(form)
+    start(b)
+    b.step(2)
+    mu_second = b.get("mu")
+    b.close()
+    for name, r, mu in (("first", r0, mu_first), ("second", r1, mu_second)):
+        ref = lat("fp64")
+        ref.init_from_rho(r)
+        ref.step(1)
+        mu64 = ref.get("mu")
+        ref.close()
+        rf = r.astype(np.float32)
+        band = (rf > lo) & (rf < hi)
+        ulps = np.abs(mu - mu64) / _ulp32(mu64)
+        print(f"[interface-mu] {form} {name}: {band.sum()} interface nodes, max {ulps[band].max():.2f} ulp; "
+              f"bulk max {ulps[~band].max():.2f} ulp", flush=True)
+        assert band.sum() > 1000
+        assert ulps[band].max() <= 1.5
+
+
 @pytest.mark.parametrize("form", ["fp64", "fp32", "fp16"])
 def test_mass_conservation_256cubed(cpmp, form):
     """Full-size property: a periodic 256^3 RK coalescence run conserves mass

@@ -150,9 +150,10 @@ def measured_traffic(kernel_key):
 
 def adjoint_cost(cpmp, nsteps=2000, every=100):
     """adjoint time over forward time, 96x96x64 sessile PR droplet, Tr = 0.75,
-    FP64, two schedules: checkpoints every `every` steps with every segment
-    recomputed, and the tape of all steps held in device memory (80 B per node
-    and step: 94 GB for 2000 steps) so that nothing is recomputed."""
+    FP64, two schedules: checkpoints every `every` steps with the segments
+    recomputed (the last one is taped during the first forward run, which its
+    working tape holds), and the tape of all steps held in device memory (80 B per
+    node and step: 94 GB for 2000 steps) so that nothing is recomputed."""
     import numpy as np
     nx, ny, nz = 96, 96, 64
     p = cpmp.make_params(nx, ny, nz, eos="PRW", tr=0.75)
@@ -327,8 +328,9 @@ def main():
     ap.add_argument("--large-every", type=int, default=100, help="checkpoint interval of the large run's adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the domain decomposition design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (the lattice "
+              "would be split into x slabs, one per GPU, with halo exchange of the "
+              "streamed populations and of rho between steps)", file=sys.stderr)
         sys.exit(2)
     cpmp = importlib.import_module("3dcpmpmodel_amd")
     n = args.size

@@ -20,7 +20,7 @@
  *   FluidCuda.cu:1561-1573 EosPressure
  *   FluidCuda.cu:1626-1657 CalcMacroCPU
  *
- * Build decisions taken where the reference is ambiguous (also in DESIGN.md):
+ * Build decisions taken where the reference is ambiguous (the same in the HIP build):
  *   - ChemBoundary: the reference kernel reads neighbour columns that other
  *     threads write in the same launch (a race).  Here the fill runs as ordered
  *     layer passes: layer nwall-1 from the first fluid layer, then each deeper

@@ -755,7 +755,7 @@ extern "C" int cpmp_adjoint(cpmp_ctx *c, const cpmp_adj_desc *d, double *J, doub
         AHIP(hipMalloc((void **)&tape, (size_t)q.tape_len * q.per_tape));
         if (q.pipe) {
             AHIP(hipMalloc((void **)&tape2, (size_t)q.tape_len * q.per_tape));
-            AHIP(hipStreamCreate(&s2));
+            AHIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
             AHIP(hipEventCreateWithFlags(&ev_first, hipEventDisableTiming));
             for (int i = 0; i < 2; ++i) {
                 AHIP(hipEventCreateWithFlags(&ev_tape[i], hipEventDisableTiming));

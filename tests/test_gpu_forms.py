@@ -32,25 +32,31 @@ def _ulp32(x):
     return (np.nextafter(a, np.float32(np.inf)) - a).astype(np.float64)
 
 
-@pytest.mark.parametrize("form", ["fp32", "fp16"])
-def test_interface_chemical_potential_in_fp64(cpmp, form):
+@pytest.mark.parametrize("form,geom", [("fp32", "periodic"), ("fp16", "periodic"), ("fp32", "walls")])
+def test_interface_chemical_potential_in_fp64(cpmp, form, geom):
     """The FP32 storage forms evaluate the EOS part of mu in FP64 at interface nodes
     (density inside the Maxwell gas-liquid gap less 2 % at each end) and round once:
     there mu matches the FP64 form's mu of the same (FP32) density field to 1.5 FP32
     ulp, both for the chemical-potential march that starts a run and for the one
     that completes the previous step's summed density (the second step of a run on a
-    periodic tiled lattice)."""
-    dims = (64, 32, 64)
-    eos, tr = "RKE", 0.6
+    periodic tiled lattice).  The walled lattice has partial tiles (13 x 36 in y, z)
+    and 3 + 3 wall layers; nodes whose Laplacian reaches the wall (within 2 layers)
+    are left out there, since the wall values are stored in FP32."""
+    if geom == "periodic":
+        dims, eos, tr, nwall, sphere = (64, 32, 64), "RKE", 0.6, 0, (32, 16, 32, 14)
+    else:
+        dims, eos, tr, nwall, sphere = (40, 13, 36), "PRW", 0.7, 3, (20, 6, 2.5, 10)
     A, B, Tc, Rc = cpmp.set_multiphase(eos, tr)
     g0, l0 = cpmp.coexistence(eos, A, B, tr * Tc)
     lo, hi = np.float32(g0 + 0.02 * (l0 - g0)), np.float32(l0 - 0.02 * (l0 - g0))
+    z = np.tile(np.arange(dims[2]), dims[0] * dims[1])
+    inner = (z >= nwall + 2) & (z < dims[2] - nwall - 2)
 
     def lat(f):
-        return cpmp.Lattice(cpmp.make_params(*dims, eos=eos, tr=tr, nwall=0, form=f, rho0=g0))
+        return cpmp.Lattice(cpmp.make_params(*dims, eos=eos, tr=tr, nwall=nwall, form=f, rho0=g0))
 
     def start(g):
-        g.init_spheres([(32, 16, 32, 14)], 5.0, g0, l0, noise_amp=1e-3, seed=5)
+        g.init_spheres([sphere], 5.0, g0, l0, noise_amp=1e-3, seed=5)
 
     a = lat(form)
     start(a)
@@ -70,11 +76,12 @@ def test_interface_chemical_potential_in_fp64(cpmp, form):
         mu64 = ref.get("mu")
         ref.close()
         rf = r.astype(np.float32)
-        band = (rf > lo) & (rf < hi)
+        band = (rf > lo) & (rf < hi) & inner
         ulps = np.abs(mu - mu64) / _ulp32(mu64)
-        print(f"[interface-mu] {form} {name}: {band.sum()} interface nodes, max {ulps[band].max():.2f} ulp; "
-              f"bulk max {ulps[~band].max():.2f} ulp", flush=True)
-        assert band.sum() > 1000
+        bulk = ~band & inner
+        print(f"[interface-mu] {form} {geom} {name}: {band.sum()} interface nodes, max {ulps[band].max():.2f} ulp; "
+              f"bulk max {np.nanmax(ulps[bulk]):.2f} ulp", flush=True)
+        assert band.sum() > 500
         assert ulps[band].max() <= 1.5
 
 

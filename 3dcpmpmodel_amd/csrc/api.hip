@@ -75,7 +75,7 @@ void fill_devparams(cpmp_ctx *c) {
     for (int q = 0; q < 19; ++q)
         if (cpmp::unit_rate_row(q) && p.Ts[q] != 1.0) d.unit_rates = 0;
     d.rho0 = p.rho0;
-    d.xc = march_chunk(p.nx, p.ny, p.nz);
+    d.xc = march_chunk(p.nx, p.ny, p.nz, p.form);
     // interface band of the FP64 EOS chemical potential (the last error is kept)
     d.band_lo = 1.0f;
     d.band_hi = 0.0f;
@@ -266,6 +266,7 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
     c->pop_bytes = pop_size(p->form);
     c->scal_bytes = scal_size(p->form);
     fill_devparams(c);
+    c->collide_v1 = node_collide(p->nx, p->ny, p->nz, p->form);
     auto fail = [&](int code, const char *what) {
         cpmp_destroy(c);
         return set_error(code, "allocation of %s failed", what);

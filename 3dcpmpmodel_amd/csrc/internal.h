@@ -61,6 +61,7 @@ struct cpmp_ctx {
     // FP32 storage forms: per wave of the chemical-potential march, a run of 64 * xc
     // slots (node, density, Laplacian term) for the interface nodes it finishes in FP64
     uint4 *band = nullptr;
+    bool collide_v1 = false;    // one-thread-per-node collide (node_collide)
     size_t fuse_bytes = 0;
     double *mv = nullptr;       // 3*N, MV of the last step
     double *mu_w = nullptr;     // 2*NXY: bottom then top
@@ -81,7 +82,11 @@ hipError_t launch_wall(cpmp_ctx *c);
 hipError_t launch_mu(cpmp_ctx *c, void *rho_copy = nullptr);
 bool density_fused(const cpmp_ctx *c);
 bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *chunk_elems);
-int march_chunk(int nx, int ny, int nz);
+// x planes per march workgroup for a lattice and storage form (CPMP_XC overrides)
+int march_chunk(int nx, int ny, int nz, int form);
+// the one-thread-per-node collide instead of the x march (CPMP_COLLIDE=v1 / march
+// override): FP64 lattices too small for 2048 march workgroups of 2 planes
+bool node_collide(int nx, int ny, int nz, int form);
 constexpr int kTileY = 8, kTileZ = 32;   // in-plane tile of the march kernels and of tiled population storage
 constexpr int kWallTile = 16;             // columns per edge of the wall-fill tiles (16 x 16 workgroups)
 // copy one population set between the device layout and the reference node order

@@ -313,7 +313,7 @@ def test_adjoint_memory_limit_sub_segments_agree(cpmp):
 
 
 @pytest.mark.parametrize("dims", [(16, 12, 20), (32, 16, 32)])
-def test_adjoint_fp32_form(cpmp, dims):
+def test_adjoint_fp32_form(cpmp, monkeypatch, dims):
     """FP32 form (32 x 16 x 32 takes the fused-density path): the reverse sweep
     runs in FP64 along the FP32 trajectory.  J and the gradients follow the FP64
     form's to the FP32 rounding of the trajectory, and every schedule (checkpoint
@@ -324,6 +324,8 @@ def test_adjoint_fp32_form(cpmp, dims):
     g.close()
     res = []
     per_tape = 40 * n           # 10 FP32 values per node
+    if dims == (32, 16, 32):
+        monkeypatch.setenv("CPMP_XC", "2")   # the fused path (the lattice alone would select single planes)
     for ck, budget, sub in ((10, 0, False), (40, 0, False), (7, 13 * per_tape, False), (40, 0, True)):
         g, target = _small_lattice(cpmp, dims, "fp32")
         limit = 0

@@ -53,11 +53,19 @@ TILED = {
 GEOMETRIES.update(TILED)
 
 
-def make(cpmp, name, form, fuse=True):
+def make(cpmp, name, form, fuse=True, xc=None):
+    """xc: planes per march chunk; the tiled geometries default to 2, which they
+    need for the fused path (lattices this small select single planes and the
+    separate density pass by themselves)"""
     spec = GEOMETRIES[name]
     old = os.environ.get("CPMP_FUSE")
+    old_xc = os.environ.get("CPMP_XC")
     if not fuse:
         os.environ["CPMP_FUSE"] = "0"
+    if xc is None and (name in TILED or name in PERIODIC_TILED):
+        xc = 2
+    if xc:
+        os.environ["CPMP_XC"] = str(xc)
     try:
         if spec is None:
             _, g = gc.build(gc.CASES[name], cpmp, form)
@@ -80,6 +88,10 @@ def make(cpmp, name, form, fuse=True):
             os.environ.pop("CPMP_FUSE", None)
         else:
             os.environ["CPMP_FUSE"] = old
+        if old_xc is None:
+            os.environ.pop("CPMP_XC", None)
+        else:
+            os.environ["CPMP_XC"] = old_xc
 
 
 @pytest.mark.parametrize("form", ["fp32", "fp16"])
@@ -117,6 +129,22 @@ def test_fused_density_matches_separate_pass(cpmp, name):
     print(f"[density] {name}: fused vs separate after 41 steps {err:.3e}", flush=True)
     assert err < 1e-4
     assert abs(a.reduce().mass - b.reduce().mass) <= 1e-6 * b.reduce().mass
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("name", list(TILED))
+def test_small_tiled_lattices_select_the_separate_pass(cpmp, name):
+    """left to choose, FP32 lattices too small for 2048 march workgroups of two
+    planes take single planes and the separate density pass (no extra buffers);
+    the density is still the sum of the pulled populations"""
+    a = make(cpmp, name, "fp32", xc=0)
+    b = make(cpmp, name, "fp32", fuse=False, xc=0)
+    assert a.memory() == b.memory()
+    fluid = a.get("type") == 1
+    a.step(3)
+    rho, s = a.get("rho"), a.get("f").reshape(19, -1).sum(axis=0)
+    assert np.max(np.abs(rho[fluid] - s[fluid])) < 2e-6 * np.max(np.abs(s[fluid]))
     a.close()
     b.close()
 

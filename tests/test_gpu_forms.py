@@ -33,7 +33,7 @@ def _ulp32(x):
 
 
 @pytest.mark.parametrize("form,geom", [("fp32", "periodic"), ("fp16", "periodic"), ("fp32", "walls")])
-def test_interface_chemical_potential_in_fp64(cpmp, form, geom):
+def test_interface_chemical_potential_in_fp64(cpmp, monkeypatch, form, geom):
     """The FP32 storage forms evaluate the EOS part of mu in FP64 at interface nodes
     (density inside the Maxwell gas-liquid gap less 2 % at each end) and round once:
     there mu matches the FP64 form's mu of the same (FP32) density field to 1.5 FP32
@@ -44,6 +44,7 @@ def test_interface_chemical_potential_in_fp64(cpmp, form, geom):
     are left out there, since the wall values are stored in FP32."""
     if geom == "periodic":
         dims, eos, tr, nwall, sphere = (64, 32, 64), "RKE", 0.6, 0, (32, 16, 32, 14)
+        monkeypatch.setenv("CPMP_XC", "2")   # the fused path with the folded completion
     else:
         dims, eos, tr, nwall, sphere = (40, 13, 36), "PRW", 0.7, 3, (20, 6, 2.5, 10)
     A, B, Tc, Rc = cpmp.set_multiphase(eos, tr)

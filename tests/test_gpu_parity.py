@@ -11,10 +11,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10   # relative, per node, on rho after N steps (FP64 form)
 
 
+@pytest.mark.parametrize("collide", ["auto", "march"])
 @pytest.mark.parametrize("name", list(gc.CASES))
-def test_fp64_rho_parity_after_n_steps(cpmp, name):
+def test_fp64_rho_parity_after_n_steps(cpmp, monkeypatch, name, collide):
+    """every case with the collide the lattice size selects (the per-node kernel
+    on these small FP64 lattices) and with the x march forced"""
     case = gc.CASES[name]
-    print(f"[parity] {name} dims={case['dims']} steps={case['steps']}", flush=True)
+    print(f"[parity] {name} dims={case['dims']} steps={case['steps']} collide={collide}", flush=True)
+    if collide == "march":
+        monkeypatch.setenv("CPMP_COLLIDE", "march")
     o, g = gc.build(case, cpmp)
     fluid = o.get(orc.TYPE) == 1
     assert gc.rel_err(g.get("rho"), o.get(orc.RHO), fluid) < 1e-13   # device tanh vs glibc

@@ -1356,20 +1356,21 @@ bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *c
 // Planes per workgroup of the march kernels: the longest chunk (32, 16, ..., 2)
 // that divides nx and still gives 2048 workgroups (8 per CU); small lattices get
 // short chunks so that they fill the GPU.  CPMP_XC overrides.
-// The longest chunk of 32..2 planes that gives 2048 workgroups.  FP32 storage forms
-// on lattices too small for 2048 workgroups of 2 planes take single planes, which leaves the fused density
-// path (it needs chunks of 2 or more) for the separate density pass: measured 9-18 %
-// faster per step at 64^3 and 96x96x64 (FP64 lattices that small take the per-node
-// collide instead, node_collide).
+// The longest chunk of 32..2 planes that gives 2048 workgroups.  FP32 storage forms on
+// lattices with fewer than 4096 workgroups of 2 planes take single planes, which leaves
+// the fused density path (it needs chunks of 2 or more) for the separate density pass:
+// measured 6-18 % faster per step at 64^3, 96x96x64 and 128x128x96, slower from 128^3 on
+// (FP64 lattices below 2048 workgroups of 2 planes take the per-node collide instead,
+// node_collide).
 int march_chunk(int nx, int ny, int nz, int form) {
     if (const char *e = std::getenv("CPMP_XC")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= MXC) return v;
     }
     const long long tiles = (long long)((ny + MTY - 1) / MTY) * ((nz + MTZ - 1) / MTZ);
+    if (form != CPMP_FORM_FP64 && (long long)((nx + 1) / 2) * tiles < 4096) return 1;   // small lattice
     for (int c = MXC; c >= 2; c /= 2)
         if (nx % c == 0 && (nx / c) * tiles >= 2048) return c;
-    if (form != CPMP_FORM_FP64 && (long long)((nx + 1) / 2) * tiles < 2048) return 1;   // small lattice
     for (int c = 2; c <= MXC; c *= 2)
         if (nx % c == 0) return c;
     for (int c = MXC; c >= 2; c /= 2)

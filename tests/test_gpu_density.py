@@ -54,9 +54,9 @@ GEOMETRIES.update(TILED)
 
 
 def make(cpmp, name, form, fuse=True, xc=None):
-    """xc: planes per march chunk; the tiled geometries default to 2, which they
-    need for the fused path (lattices this small select single planes and the
-    separate density pass by themselves)"""
+    """xc: planes per march chunk (0: the library's choice); the tiled geometries
+    default to 2, which they need for the fused path (lattices this small select
+    single planes and the separate density pass by themselves)"""
     spec = GEOMETRIES[name]
     old = os.environ.get("CPMP_FUSE")
     old_xc = os.environ.get("CPMP_XC")
@@ -66,6 +66,8 @@ def make(cpmp, name, form, fuse=True, xc=None):
         xc = 2
     if xc:
         os.environ["CPMP_XC"] = str(xc)
+    elif xc == 0:              # the library's own choice
+        os.environ.pop("CPMP_XC", None)
     try:
         if spec is None:
             _, g = gc.build(gc.CASES[name], cpmp, form)

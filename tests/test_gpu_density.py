@@ -62,6 +62,8 @@ def make(cpmp, name, form, fuse=True, xc=None):
     old_xc = os.environ.get("CPMP_XC")
     if not fuse:
         os.environ["CPMP_FUSE"] = "0"
+    else:                      # the fused path where the geometry allows it
+        os.environ.pop("CPMP_FUSE", None)
     if xc is None and (name in TILED or name in PERIODIC_TILED):
         xc = 2
     if xc:

@@ -645,7 +645,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
 // rings of the neighbouring tiles and the partial planes of the neighbouring
 // chunks, in a fixed order, with no atomics to global memory.  Requires
 // ny % MTY == nz % MTZ == nx % P.xc == 0 (fuse_geometry).
-template <class TF, class TS, class TC, bool MV, bool RS, bool UR>
+template <class TF, class TS, class TC, bool MV, bool RS, bool UR, bool NW>
 __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
@@ -668,7 +668,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     const int xa = ch * P.xc, xb = min(xa + P.xc, P.nx);
     const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
     const int y = y0 + ty, z = z0 + tz;
-    const bool active = (y < P.ny) && (z < P.nz) && !is_solid(P, z);
+    const bool active = (y < P.ny) && (z < P.nz) && (NW || !is_solid(P, z));
 
     // Halo entries of one plane owned by this thread (e = tid + h*MT) and their
     // in-plane offsets; the plane index is added per x.
@@ -700,6 +700,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     // populations arriving at (xg, ym, zm), gathered from their emitting nodes;
     // the in-plane neighbour offsets and wall flags do not change along x
     Nbr nb = make_nbr(P, xa, ym, zm);
+    if (NW) nb.sb[0] = nb.sb[2] = false;   // no solid layers: nothing bounces back
     auto gather = [&](int xg, TC *dst) {
         xg = wrapc(xg, P.nx);
         nb.site = (unsigned)xg * plane + own_site;
@@ -727,7 +728,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     const int own = (ty + 1) * AZ + tz + 1;
     const int rpos = ring_pos(threadIdx.x < ARING ? threadIdx.x : 0);
     const int rly = rpos / AZ, rlz = rpos - rly * AZ;
-    const bool bb_m = is_solid(P, wrap_off(zm, -1, P.nz)), bb_p = is_solid(P, wrap_off(zm, 1, P.nz));
+    const bool bb_m = !NW && is_solid(P, wrap_off(zm, -1, P.nz)), bb_p = !NW && is_solid(P, wrap_off(zm, 1, P.nz));
     TC acc_prev = TC(0), acc_cur = TC(0), acc_next = TC(0);
     TC rng_prev = TC(0), rng_cur = TC(0), rng_next = TC(0);
     if (RS && !active) {
@@ -1454,11 +1455,17 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sum
             const dim3 grid((unsigned)(ntz * nty * nch));
             constexpr bool can_rs = !std::is_same<TF, double>::value;   // FP64 keeps the separate pass
             // MV, RS and the unit-rate specialisation as template arguments
+            // NW: no solid layers (lo = hi = 0), the bounce-back selects compile away
+            const bool nw = c->dp.lo == 0 && c->dp.hi == 0;
             auto go = [&](auto mv_t, auto rs_t, auto ur_t) {
-                hipLaunchKernelGGL((k_collide_march<TF, TS, TC, decltype(mv_t)::value, decltype(rs_t)::value,
-                                                    decltype(ur_t)::value>),
-                                   grid, dim3(MT), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ntz, nty, rn, rb,
-                                   cb, remap, ref_buffer<TS>(c));
+                auto launch = [&](auto nw_t) {
+                    hipLaunchKernelGGL((k_collide_march<TF, TS, TC, decltype(mv_t)::value, decltype(rs_t)::value,
+                                                        decltype(ur_t)::value, decltype(nw_t)::value>),
+                                       grid, dim3(MT), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ntz, nty, rn,
+                                       rb, cb, remap, ref_buffer<TS>(c));
+                };
+                if (nw) launch(std::true_type{});
+                else launch(std::false_type{});
             };
             using T1 = std::true_type;
             using F0 = std::false_type;

@@ -645,7 +645,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
 // rings of the neighbouring tiles and the partial planes of the neighbouring
 // chunks, in a fixed order, with no atomics to global memory.  Requires
 // ny % MTY == nz % MTZ == nx % P.xc == 0 (fuse_geometry).
-template <class TF, class TS, class TC, bool MV, bool RS, bool UR, bool NW>
+template <class TF, class TS, class TC, bool MV, bool RS, bool UR, bool NW, bool QPR>
 __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
@@ -804,7 +804,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
             const TC third = TC(1.0 / 3.0);
             const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
             TC out[Q], u[3];
-            if (P.qp_on) {
+            if (QPR && P.qp_on) {   // QPR = false: the Qp term is off (K1 = K2 = 0), compiled out
                 TC qp[Q];
                 qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
                 collide<TC, UR>(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
@@ -1457,15 +1457,23 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sum
             // MV, RS and the unit-rate specialisation as template arguments
             // NW: no solid layers (lo = hi = 0), the bounce-back selects compile away
             const bool nw = c->dp.lo == 0 && c->dp.hi == 0;
+            // QPR: the Qp term may be on (always in the MV launches, rare)
+            const bool qp_on = c->dp.qp_on != 0;
             auto go = [&](auto mv_t, auto rs_t, auto ur_t) {
-                auto launch = [&](auto nw_t) {
+                auto launch = [&](auto nw_t, auto qp_t) {
                     hipLaunchKernelGGL((k_collide_march<TF, TS, TC, decltype(mv_t)::value, decltype(rs_t)::value,
-                                                        decltype(ur_t)::value, decltype(nw_t)::value>),
+                                                        decltype(ur_t)::value, decltype(nw_t)::value,
+                                                        decltype(qp_t)::value>),
                                        grid, dim3(MT), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ntz, nty, rn,
                                        rb, cb, remap, ref_buffer<TS>(c));
                 };
-                if (nw) launch(std::true_type{});
-                else launch(std::false_type{});
+                auto pick = [&](auto nw_t) {
+                    if constexpr (decltype(mv_t)::value) launch(nw_t, std::true_type{});
+                    else if (qp_on) launch(nw_t, std::true_type{});
+                    else launch(nw_t, std::false_type{});
+                };
+                if (nw) pick(std::true_type{});
+                else pick(std::false_type{});
             };
             using T1 = std::true_type;
             using F0 = std::false_type;

@@ -1457,7 +1457,9 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sum
             // MV, RS and the unit-rate specialisation as template arguments
             // NW: no solid layers (lo = hi = 0), the bounce-back selects compile away
             const bool nw = c->dp.lo == 0 && c->dp.hi == 0;
-            // QPR: the Qp term may be on (always in the MV launches, rare)
+            // QPR: the Qp term may be on (always in the MV launches, which are rare, and in
+            // FP64, whose collide measured 4 % slower without the Qp path: 234 VGPRs give it
+            // 2 waves per SIMD instead of 1, and the second wave costs more than it hides)
             const bool qp_on = c->dp.qp_on != 0;
             auto go = [&](auto mv_t, auto rs_t, auto ur_t) {
                 auto launch = [&](auto nw_t, auto qp_t) {
@@ -1468,7 +1470,7 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sum
                                        rb, cb, remap, ref_buffer<TS>(c));
                 };
                 auto pick = [&](auto nw_t) {
-                    if constexpr (decltype(mv_t)::value) launch(nw_t, std::true_type{});
+                    if constexpr (decltype(mv_t)::value || std::is_same<TF, double>::value) launch(nw_t, std::true_type{});
                     else if (qp_on) launch(nw_t, std::true_type{});
                     else launch(nw_t, std::false_type{});
                 };

@@ -32,7 +32,7 @@ EXPORTS = [
     "cpmp_laplace_jump",
     "cpmp_checkpoint_save", "cpmp_checkpoint_load", "cpmp_write_plt", "cpmp_adjoint", "cpmp_adjoint_plan", "cpmp_neck_radius",
     "cpmp_kernel_times", "cpmp_time_steps", "cpmp_algorithmic_bytes_per_node",
-    "cpmp_last_error", "cpmp_version", "cpmp_selftest_collide",
+    "cpmp_last_error", "cpmp_version", "cpmp_selftest_collide", "cpmp_selftest_collide_guo",
 ]
 
 
@@ -43,7 +43,7 @@ class Params(C.Structure):
                 ("wall_scheme", C.c_int), ("form", C.c_int), ("device", C.c_int),
                 ("A", C.c_double), ("B", C.c_double), ("T", C.c_double),
                 ("K", C.c_double), ("Ka", C.c_double), ("K1", C.c_double), ("K2", C.c_double),
-                ("Ts", C.c_double * 19), ("rho0", C.c_double)]
+                ("Ts", C.c_double * 19), ("rho0", C.c_double), ("forcing", C.c_int)]
 
 
 class InitDesc(C.Structure):
@@ -129,6 +129,7 @@ def _load():
     L.cpmp_last_error.restype = C.c_char_p
     L.cpmp_version.restype = C.c_char_p
     L.cpmp_selftest_collide.argtypes = [p, d, p, p, p, p, p]
+    L.cpmp_selftest_collide_guo.argtypes = [p, d, p, p, p, p, p]
     return L
 
 
@@ -164,7 +165,7 @@ def reference_ts(tau):
 
 def make_params(nx, ny, nz, eos="PRW", tr=0.7, tau=0.8, K=0.2, Ka=0.001, K1=0.0, K2=0.0,
                 nwall=3, nwall_hi=None, mu_form=0, lap_points=5, wall_scheme=1, form="fp64",
-                rho0=0.0, device=0, ts=None):
+                rho0=0.0, device=0, ts=None, forcing=0):
     p = Params()
     check(lib.cpmp_default_params(C.byref(p), nx, ny, nz, EOS[eos], tr))
     p.nwall_lo = nwall
@@ -176,6 +177,7 @@ def make_params(nx, ny, nz, eos="PRW", tr=0.7, tau=0.8, K=0.2, Ka=0.001, K1=0.0,
     for q, v in enumerate(ts if ts is not None else reference_ts(tau)):
         p.Ts[q] = v
     p.rho0 = rho0
+    p.forcing = forcing
     return p
 
 
@@ -206,15 +208,16 @@ def eos_mu(eos, A, B, T, rho, mu_form=0):
     return lib.cpmp_eos_mu(EOS[eos], mu_form, A, B, T, rho)
 
 
-def selftest_collide(f, rho, F, ts, qp=None):
-    """The device collision of one node evaluated on the CPU: (out, u)."""
+def selftest_collide(f, rho, F, ts, qp=None, forcing=0):
+    """The device collision of one node evaluated on the CPU: (out, u); forcing 1:
+    the Guo force term instead of the exact-difference one."""
     f = np.ascontiguousarray(f, np.float64)
     F = np.ascontiguousarray(F, np.float64)
     ts = np.ascontiguousarray(ts, np.float64)
     out, u = np.zeros(19), np.zeros(3)
     qpp = None if qp is None else np.ascontiguousarray(qp, np.float64)
-    check(lib.cpmp_selftest_collide(_ptr(f), rho, _ptr(F), _ptr(ts),
-                                    None if qpp is None else _ptr(qpp), _ptr(out), _ptr(u)))
+    fn = lib.cpmp_selftest_collide_guo if forcing == 1 else lib.cpmp_selftest_collide
+    check(fn(_ptr(f), rho, _ptr(F), _ptr(ts), None if qpp is None else _ptr(qpp), _ptr(out), _ptr(u)))
     return out, u
 
 

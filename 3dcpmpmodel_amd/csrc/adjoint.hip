@@ -608,6 +608,8 @@ int make_plan(cpmp_ctx *c, const cpmp_adj_desc *d, AdjPlan &q) {
     if (c->form != CPMP_FORM_FP64 && c->form != CPMP_FORM_FP32)
         return set_error(CPMP_EINVAL, "the adjoint runs in the FP64 and FP32 forms");
     if (d->nsteps < 1 || d->checkpoint_every < 1) return set_error(CPMP_EINVAL, "nsteps and checkpoint_every must be >= 1");
+    if (c->p.forcing != 0) return set_error(CPMP_EINVAL, "the adjoint is derived for the exact-difference force term");
+    if (c->p.wall_scheme == 2) return set_error(CPMP_EINVAL, "the adjoint covers the layered and simple wall schemes");
     const long long N = c->N, NXY = c->NXY;
     const dim3 ng = node_grid(c->dp);
     const long long nb = blocks_for(N), nbg = (long long)ng.x * ng.y * ng.z;

@@ -75,6 +75,7 @@ void fill_devparams(cpmp_ctx *c) {
     for (int q = 0; q < 19; ++q)
         if (cpmp::unit_rate_row(q) && p.Ts[q] != 1.0) d.unit_rates = 0;
     d.rho0 = p.rho0;
+    d.forcing = p.forcing;
     d.xc = march_chunk(p.nx, p.ny, p.nz, p.form);
     // interface band of the FP64 EOS chemical potential (the last error is kept)
     d.band_lo = 1.0f;
@@ -103,6 +104,8 @@ int check_params(const cpmp_params *p) {
     if (p->eos < 0 || p->eos > 5) return set_error(CPMP_EINVAL, "eos id %d", p->eos);
     if (p->lap_points != 5 && p->lap_points != 7) return set_error(CPMP_EINVAL, "lap_points must be 5 or 7");
     if (p->form < 0 || p->form > 2) return set_error(CPMP_EINVAL, "form %d", p->form);
+    if (p->wall_scheme < 0 || p->wall_scheme > 2) return set_error(CPMP_EINVAL, "wall_scheme %d", p->wall_scheme);
+    if (p->forcing != 0 && p->forcing != 1) return set_error(CPMP_EINVAL, "forcing %d", p->forcing);
     if (!(p->B > 0) || !(p->T > 0)) return set_error(CPMP_EINVAL, "EOS constants B=%g T=%g", p->B, p->T);
     for (int q = 0; q < 19; ++q)
         if (!(p->Ts[q] >= 0 && p->Ts[q] < 2)) return set_error(CPMP_EINVAL, "Ts[%d]=%g outside [0,2)", q, p->Ts[q]);
@@ -677,7 +680,13 @@ double cpmp_algorithmic_bytes_per_node(cpmp_ctx *c) {
 int cpmp_selftest_collide(const double *f, double rho, const double *F, const double *ts, const double *qp,
                           double *out, double *u) {
     if (!f || !F || !ts || !out || !u) return set_error(CPMP_EINVAL, "NULL argument");
-    host_collide_fp64(f, rho, F, ts, qp, out, u);
+    host_collide_fp64(f, rho, F, ts, qp, out, u, 0);
+    return CPMP_OK;
+}
+int cpmp_selftest_collide_guo(const double *f, double rho, const double *F, const double *ts, const double *qp,
+                              double *out, double *u) {
+    if (!f || !F || !ts || !out || !u) return set_error(CPMP_EINVAL, "NULL argument");
+    host_collide_fp64(f, rho, F, ts, qp, out, u, 1);
     return CPMP_OK;
 }
 

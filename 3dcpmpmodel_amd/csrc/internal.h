@@ -26,6 +26,7 @@ struct DevParams {
     int tiled, nty, ntz;        // tile-contiguous population sites (pop_site), tile counts in y, z
     int xc;                     // x planes per workgroup of the march kernels
     int unit_rates;             // the rates of rows 0, 1, 3, 5, 7, 10, 12, 16-18 are all 1 (reference set)
+    int forcing;                // 0 exact-difference force term, 1 Guo
     // FP32 storage forms: fluid nodes with a density strictly inside (band_lo, band_hi)
     // -- the interface: between the Maxwell gas and liquid densities, less 2 % of their
     // gap at each end -- get the EOS chemical potential in FP64; empty (lo > hi) without
@@ -121,6 +122,6 @@ size_t reduce_scratch_doubles();
 
 // host-side CPU evaluation of the per-node collision (self-test hook)
 void host_collide_fp64(const double f[19], double rho, const double F[3], const double ts[19],
-                       const double *qp, double out[19], double u[3]);
+                       const double *qp, double out[19], double u[3], int forcing);
 
 }  // namespace cpmp

@@ -71,6 +71,31 @@ __device__ __forceinline__ TS ref_at(const TS *rref, long long I) {
 // Pass p writes bottom layer lo-1-p and top layer nz-hi+p from the layer next to
 // it (layered scheme) or from the first fluid layer (simple scheme).  Pass 0 also
 // writes the wall chemical potential into every solid node of the column.
+//
+// Scheme 2 is ChemBoundaryComplex (FluidCuda.cu:403-429, launched per level at
+// :1281-1283): a solid node at distance d from the fluid (tag LEVELd, Common.h:40-42;
+// fluid is level 0) gets sum_f Alpha_f rho(x + e_f) / sum_f Alpha_f over the
+// directions f = 1..18, in that order, whose neighbour (periodic in x, y, z) has level
+// d - 1.  Pass p fills level p + 1 of both walls; it reads only level p nodes, which
+// no thread of the pass writes.
+__device__ __forceinline__ int wall_level(const DevParams &P, int z) {
+    return z < P.lo ? P.lo - z : (z >= P.nz - P.hi ? z - (P.nz - P.hi) + 1 : 0);
+}
+template <class TS>
+__device__ __forceinline__ double level_average(const DevParams &P, const TS *rho, int x, int y, int z, int level) {
+    double s = 0, w = 0;
+#pragma unroll
+    for (int f = 1; f < Q; ++f) {
+        if (cz(f) == 0) continue;   // same z: the same level, never level - 1
+        const int zz = wrapc(z + cz(f), P.nz);
+        if (wall_level(P, zz) != level - 1) continue;
+        const int xx = wrapc(x + cx(f), P.nx), yy = wrapc(y + cy(f), P.ny);
+        s += wq(f) * (double)rho[((long long)xx * P.ny + yy) * P.nz + zz];
+        w += wq(f);
+    }
+    return s / w;
+}
+
 template <class TS>
 __global__ void __launch_bounds__(TPB) k_wall(const DevParams P, TS *__restrict__ rho, TS *__restrict__ mu,
                                               const double *__restrict__ muw, int pass) {
@@ -84,13 +109,18 @@ __global__ void __launch_bounds__(TPB) k_wall(const DevParams P, TS *__restrict_
         return ((double)rho[at(x, y, k)] * 2 + (double)rho[at(x1, y1, k)] + (double)rho[at(x1, y2, k)]
                 + (double)rho[at(x2, y1, k)] + (double)rho[at(x2, y2, k)]) / 6;
     };
-    if (pass < P.lo) {
-        const int kd = P.lo - 1 - pass, ks = P.wall_scheme ? kd + 1 : P.lo;
-        rho[at(x, y, kd)] = (TS)avg(ks);
-    }
-    if (pass < P.hi) {
-        const int kd = P.nz - P.hi + pass, ks = P.wall_scheme ? kd - 1 : P.nz - 1 - P.hi;
-        rho[at(x, y, kd)] = (TS)avg(ks);
+    if (P.wall_scheme == 2) {
+        if (pass < P.lo) rho[at(x, y, P.lo - 1 - pass)] = (TS)level_average(P, rho, x, y, P.lo - 1 - pass, pass + 1);
+        if (pass < P.hi) rho[at(x, y, P.nz - P.hi + pass)] = (TS)level_average(P, rho, x, y, P.nz - P.hi + pass, pass + 1);
+    } else {
+        if (pass < P.lo) {
+            const int kd = P.lo - 1 - pass, ks = P.wall_scheme ? kd + 1 : P.lo;
+            rho[at(x, y, kd)] = (TS)avg(ks);
+        }
+        if (pass < P.hi) {
+            const int kd = P.nz - P.hi + pass, ks = P.wall_scheme ? kd - 1 : P.nz - 1 - P.hi;
+            rho[at(x, y, kd)] = (TS)avg(ks);
+        }
     }
     if (pass == 0) {
         const double ml = muw[col], mh = muw[nxy + col];
@@ -280,7 +310,8 @@ __global__ void __launch_bounds__(TPB) k_mu(const DevParams P, const TS *__restr
 // Fused non-ideal force + MRT collision + exact-difference forcing + push
 // streaming with half-way bounce-back (NonidealForce + GlobalCollide,
 // FluidCuda.cu:311-347, 431-441).  u = j(f)/rho with rho the stored density.
-template <class TF, class TS, class TC, bool MV>
+// GUO: the Guo force term instead of the exact-difference one (collide()).
+template <class TF, class TS, class TC, bool MV, bool GUO = false>
 __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__restrict__ fin, TF *__restrict__ fout,
                                                  const TS *__restrict__ rho, const TS *__restrict__ mu,
                                                  double *__restrict__ mv, TS *__restrict__ uout,
@@ -318,9 +349,9 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
     if (P.qp_on) {
         TC qp[Q];
         qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
-        collide(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
+        collide<TC, false, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
     } else {
-        collide(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
+        collide<TC, false, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
     }
     if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
         const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
@@ -645,7 +676,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
 // rings of the neighbouring tiles and the partial planes of the neighbouring
 // chunks, in a fixed order, with no atomics to global memory.  Requires
 // ny % MTY == nz % MTZ == nx % P.xc == 0 (fuse_geometry).
-template <class TF, class TS, class TC, bool MV, bool RS, bool UR, bool NW, bool QPR>
+template <class TF, class TS, class TC, bool MV, bool RS, bool UR, bool NW, bool QPR, bool GUO>
 __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const TF *__restrict__ fin,
                                                        TF *__restrict__ fout, const TS *__restrict__ rho,
                                                        const TS *__restrict__ mu, double *__restrict__ mv,
@@ -807,9 +838,9 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
             if (QPR && P.qp_on) {   // QPR = false: the Qp term is off (K1 = K2 = 0), compiled out
                 TC qp[Q];
                 qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
-                collide<TC, UR>(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
+                collide<TC, UR, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
             } else {
-                collide<TC, UR>(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
+                collide<TC, UR, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
             }
             if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
                 const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
@@ -1322,7 +1353,10 @@ hipError_t launch_wall(cpmp_ctx *c) {
         const dim3 g(blocks_for(c->NXY));
         const dim3 gt((P.nx + kWallTile - 1) / kWallTile, (P.ny + kWallTile - 1) / kWallTile, 2), bt(kWallTile * kWallTile);
         TS *rho = (TS *)c->rho, *mu = (TS *)c->mu;
-        if (passes == 1) hipLaunchKernelGGL((k_wall_tile<1, TS>), gt, bt, 0, c->stream, P, rho, mu, c->mu_w);
+        if (P.wall_scheme == 2)   // ChemBoundaryComplex: one launch per level, as the reference
+            for (int p = 0; p < passes; ++p)
+                hipLaunchKernelGGL(k_wall<TS>, g, dim3(TPB), 0, c->stream, P, rho, mu, c->mu_w, p);
+        else if (passes == 1) hipLaunchKernelGGL((k_wall_tile<1, TS>), gt, bt, 0, c->stream, P, rho, mu, c->mu_w);
         else if (passes == 2) hipLaunchKernelGGL((k_wall_tile<2, TS>), gt, bt, 0, c->stream, P, rho, mu, c->mu_w);
         else if (passes == 3) hipLaunchKernelGGL((k_wall_tile<3, TS>), gt, bt, 0, c->stream, P, rho, mu, c->mu_w);
         else   // deeper walls: one launch per layer
@@ -1443,13 +1477,14 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sum
             const char *e = std::getenv("CPMP_REMAP");
             return (e && std::strcmp(e, "1") == 0) ? 1 : 0;
         }();
+        const bool guo = c->dp.forcing == 1;
         if (c->collide_v1) {
-            if (want_mv)
-                hipLaunchKernelGGL((k_collide<TF, TS, TC, true>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ref_buffer<TS>(c));
-            else
-                hipLaunchKernelGGL((k_collide<TF, TS, TC, false>), node_grid(c->dp), dim3(TPB), 0, c->stream, c->dp,
-                                   fin, fout, rho, mu, c->mv, uout, ref_buffer<TS>(c));
+            auto v1 = [&](auto mv_t, auto guo_t) {
+                hipLaunchKernelGGL((k_collide<TF, TS, TC, decltype(mv_t)::value, decltype(guo_t)::value>), node_grid(c->dp),
+                                   dim3(TPB), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ref_buffer<TS>(c));
+            };
+            if (guo) { if (want_mv) v1(std::true_type{}, std::true_type{}); else v1(std::false_type{}, std::true_type{}); }
+            else { if (want_mv) v1(std::true_type{}, std::false_type{}); else v1(std::false_type{}, std::false_type{}); }
         } else {
             const int ntz = (c->p.nz + MTZ - 1) / MTZ, nty = (c->p.ny + MTY - 1) / MTY, nch = (c->p.nx + c->dp.xc - 1) / c->dp.xc;
             const dim3 grid((unsigned)(ntz * nty * nch));
@@ -1462,17 +1497,20 @@ hipError_t launch_collide(cpmp_ctx *c, bool want_mv, void *uout_v, bool keep_sum
             // 2 waves per SIMD instead of 1, and the second wave costs more than it hides)
             const bool qp_on = c->dp.qp_on != 0;
             auto go = [&](auto mv_t, auto rs_t, auto ur_t) {
-                auto launch = [&](auto nw_t, auto qp_t) {
+                auto launch = [&](auto nw_t, auto qp_t, auto guo_t) {
                     hipLaunchKernelGGL((k_collide_march<TF, TS, TC, decltype(mv_t)::value, decltype(rs_t)::value,
                                                         decltype(ur_t)::value, decltype(nw_t)::value,
-                                                        decltype(qp_t)::value>),
+                                                        decltype(qp_t)::value, decltype(guo_t)::value>),
                                        grid, dim3(MT), 0, c->stream, c->dp, fin, fout, rho, mu, c->mv, uout, ntz, nty, rn,
                                        rb, cb, remap, ref_buffer<TS>(c));
                 };
+                // Guo forcing: one specialisation per (MV, RS, UR, NW), with the Qp path
                 auto pick = [&](auto nw_t) {
-                    if constexpr (decltype(mv_t)::value || std::is_same<TF, double>::value) launch(nw_t, std::true_type{});
-                    else if (qp_on) launch(nw_t, std::true_type{});
-                    else launch(nw_t, std::false_type{});
+                    if (guo) launch(nw_t, std::true_type{}, std::true_type{});
+                    else if constexpr (decltype(mv_t)::value || std::is_same<TF, double>::value)
+                        launch(nw_t, std::true_type{}, std::false_type{});
+                    else if (qp_on) launch(nw_t, std::true_type{}, std::false_type{});
+                    else launch(nw_t, std::false_type{}, std::false_type{});
                 };
                 if (nw) pick(std::true_type{});
                 else pick(std::false_type{});
@@ -1740,8 +1778,9 @@ size_t reduce_scratch_doubles() { return (size_t)(RED_BLOCKS + 1) * RED_VALS; }
 
 // host evaluation of the collision (same code as the device path, FP64)
 void host_collide_fp64(const double f[19], double rho, const double F[3], const double ts[19], const double *qp,
-                       double out[19], double u[3]) {
-    collide(f, rho, F[0], F[1], F[2], ts, qp, out, u);
+                       double out[19], double u[3], int forcing) {
+    if (forcing == 1) collide<double, false, true>(f, rho, F[0], F[1], F[2], ts, qp, out, u);
+    else collide(f, rho, F[0], F[1], F[2], ts, qp, out, u);
 }
 
 }  // namespace cpmp

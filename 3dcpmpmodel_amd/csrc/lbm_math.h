@@ -199,17 +199,25 @@ CPMP_HD T recip(T x) {
 
 // UR: the rates of the unit_rate_row rows are 1 (the caller has checked); those rows
 // skip (I - S)(m - Meq), which is exactly 0, and their moments are not formed.
-template <class T, bool UR = false>
+//
+// GUO: Guo forcing (Guo, Zheng and Shi, Phys. Rev. E 65, 046308 (2002)) in its MRT
+// form instead of the exact-difference term:
+//   g  = (j + F/2) / rho
+//   m* = m - S(m - Meq(rho,g)) + (I - S/2) G + S Qp,   out = R m*
+// with G = M Fbar, Fbar_q = w_q [3 (e_q - g) + 9 (e_q.g) e_q].F.  Since Fbar is the
+// derivative of Feq(rho, g) along F/rho and M Feq = Meq, G is the derivative of the
+// Meq rows along F/rho: G0 = 0, G1 = 38 g.F, G2 = -11 g.F, G3 = Fx, G4 = -2/3 Fx
+// (y, z alike), G9 = 2 (2 gx Fx - gy Fy - gz Fz), G10 = -G9/2, G11 = 2 (gy Fy - gz Fz),
+// G12 = -G11/2, G13 = gx Fy + gy Fx, G14 = gy Fz + gz Fy, G15 = gx Fz + gz Fx,
+// G16..18 = 0.  Mass is conserved and the momentum gains F exactly.  u returns j/rho
+// in both forms.
+template <class T, bool UR = false, bool GUO = false>
 CPMP_HD void collide(const T f[Q], T rho, T Fx, T Fy, T Fz, const T *ts, const T *qp, T out[Q], T u[3]) {
     T m[Q];
     moments(f, m);
     const T inv = recip(rho);
     const T ux = m[3] * inv, uy = m[5] * inv, uz = m[7] * inv;
     u[0] = ux; u[1] = uy; u[2] = uz;
-    const T ex = ux + Fx * inv, ey = uy + Fy * inv, ez = uz + Fz * inv;
-    const T uu = ux * ux + uy * uy + uz * uz, ee = ex * ex + ey * ey + ez * ez;
-    const T u9 = T(2) * ux * ux - uy * uy - uz * uz, e9 = T(2) * ex * ex - ey * ey - ez * ez;
-    const T u11 = uy * uy - uz * uz, e11 = ey * ey - ez * ez;
     T a[Q];
     auto row = [&](int r, T mequ, T meqe) {
         T v = (UR && unit_rate_row(r)) ? meqe : (T(1) - ts[r]) * (m[r] - mequ) + meqe;
@@ -217,6 +225,39 @@ CPMP_HD void collide(const T f[Q], T rho, T Fx, T Fy, T Fz, const T *ts, const T
         a[r] = v * T(1.0 / n2(r));
     };
     const T c23 = T(-2.0 / 3.0);
+    if constexpr (GUO) {
+        const T h = T(0.5) * inv;
+        const T gx = ux + Fx * h, gy = uy + Fy * h, gz = uz + Fz * h;
+        const T gg = gx * gx + gy * gy + gz * gz, gF = gx * Fx + gy * Fy + gz * Fz;
+        const T g9 = T(2) * gx * gx - gy * gy - gz * gz, g11 = gy * gy - gz * gz;
+        const T G9 = T(2) * (T(2) * gx * Fx - gy * Fy - gz * Fz), G11 = T(2) * (gy * Fy - gz * Fz);
+        // row r: Meq_r(g) and Meq_r(g) + (1 - s_r/2) G_r
+        auto grow = [&](int r, T mq, T G) { row(r, mq, mq + (T(1) - T(0.5) * ts[r]) * G); };
+        row(0, rho, rho);
+        grow(1, rho * (T(-11) + T(19) * gg), T(38) * gF);
+        grow(2, rho * (T(3) - T(5.5) * gg), T(-11) * gF);
+        grow(3, rho * gx, Fx);
+        grow(4, rho * gx * c23, Fx * c23);
+        grow(5, rho * gy, Fy);
+        grow(6, rho * gy * c23, Fy * c23);
+        grow(7, rho * gz, Fz);
+        grow(8, rho * gz * c23, Fz * c23);
+        grow(9, rho * g9, G9);
+        grow(10, rho * g9 * T(-0.5), G9 * T(-0.5));
+        grow(11, rho * g11, G11);
+        grow(12, rho * g11 * T(-0.5), G11 * T(-0.5));
+        grow(13, rho * gx * gy, gx * Fy + gy * Fx);
+        grow(14, rho * gy * gz, gy * Fz + gz * Fy);
+        grow(15, rho * gx * gz, gx * Fz + gz * Fx);
+#pragma unroll
+        for (int r = 16; r < Q; ++r) a[r] = UR ? T(0) : (T(1) - ts[r]) * m[r] * T(1.0 / n2(r));
+        inverse_scaled<T, UR>(a, out);
+        return;
+    }
+    const T ex = ux + Fx * inv, ey = uy + Fy * inv, ez = uz + Fz * inv;
+    const T uu = ux * ux + uy * uy + uz * uz, ee = ex * ex + ey * ey + ez * ez;
+    const T u9 = T(2) * ux * ux - uy * uy - uz * uz, e9 = T(2) * ex * ex - ey * ey - ez * ez;
+    const T u11 = uy * uy - uz * uz, e11 = ey * ey - ez * ez;
     row(0, rho, rho);
     row(1, rho * (T(-11) + T(19) * uu), rho * (T(-11) + T(19) * ee));
     row(2, rho * (T(3) - T(5.5) * uu), rho * (T(3) - T(5.5) * ee));

@@ -26,13 +26,18 @@ extern "C" {
 // ContactAngle(side=0, tier), FluidCuda.cu:557-689.  Interface x on rows z=2..5 by
 // linear interpolation at (rhoG+rhoL)/2 (:595-623), linear extrapolation of rows
 // z=3,4 to z=2 (:634-637), angles from the extrapolated foot (:641-649), and the
-// spherical-cap angle from height H and base L (:656-680).  When no interface
-// crosses the last row scanned for H, H is NaN (the reference leaves it unset).
+// spherical-cap angle from height H and base L (:656-680).  The reference rows fit
+// its three bottom wall layers (z = 2 is the last solid layer); the rows here are
+// offset by the bottom wall depth (begin = nwall_lo - 1), and a lattice without a
+// bottom wall is rejected.  When no interface crosses the last row scanned for H,
+// H is NaN (the reference leaves it unset).
 int cpmp_contact_angle(cpmp_ctx *c, int tier, double rho_g, double rho_l, double *out) {
     if (!c || !out) return set_error(CPMP_EINVAL, "bad context or output");
     const int DX = c->p.nx, DY = c->p.ny, DZ = c->p.nz;
     if (tier < 0 || tier >= DY) return set_error(CPMP_EINVAL, "tier %d outside [0,%d)", tier, DY);
     if (DX < 8 || DZ < 8) return set_error(CPMP_EINVAL, "domain too small for the contact-angle scan");
+    const int begin = c->p.nwall_lo - 1;   // last solid layer: the row the foot is extrapolated to
+    if (begin < 0 || begin + 5 >= DZ) return set_error(CPMP_EINVAL, "the contact-angle scan needs a bottom wall (nwall_lo=%d)", c->p.nwall_lo);
     OBS_HIP(hipSetDevice(c->p.device));
     std::vector<double> plane((size_t)DX * DZ);
     OBS_HIP(launch_plane_y(c, tier, c->staging));
@@ -42,8 +47,8 @@ int cpmp_contact_angle(cpmp_ctx *c, int tier, double rho_g, double rho_l, double
     const double Mid = (rho_g + rho_l) / 2;
     auto T = [&](int i, int k) { return plane[(size_t)i * DZ + k]; };
     double Px[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int j = 2; j <= 5; ++j) {
-        const int n = (j - 2) * 2;
+    for (int j = begin; j <= begin + 3; ++j) {
+        const int n = (j - begin) * 2;
         for (int i = 0; i < DX / 2 + 2; ++i) {
             const double t1 = T(i + 1, j), t2 = T(i + 2, j);
             if (t1 < Mid && t2 >= Mid) { Px[n] = double(i + 1) + (t1 - Mid) / (t1 - t2); break; }
@@ -53,7 +58,7 @@ int cpmp_contact_angle(cpmp_ctx *c, int tier, double rho_g, double rho_l, double
             if (t1 < Mid && t2 >= Mid) { Px[n + 1] = double(i - 1) - (t1 - Mid) / (t1 - t2); break; }
         }
     }
-    const double y0 = 2, y1 = 3, y2 = 4;
+    const double y0 = begin, y1 = begin + 1, y2 = begin + 2;
     double CaLeft = 0, CaRight = 0, CaCap = 0, XLeft = DY / 2, XRight = DY / 2, H = NAN, L = 0;
     int valid = 0;
     if (Px[0] != 0.0 && Px[1] != 0.0) {
@@ -69,7 +74,7 @@ int cpmp_contact_angle(cpmp_ctx *c, int tier, double rho_g, double rho_l, double
         L = XRight - XLeft;
         const int ic = int((Px[4] + Px[5]) / 2 + 0.5);
         if (ic >= 0 && ic < DX)
-            for (int j = 2; j < DZ - 2; ++j) {
+            for (int j = begin; j < DZ - 2; ++j) {
                 const double t = T(ic, j), t1 = T(ic, j + 1);
                 if (t >= Mid && t1 < Mid) H = double(j) + (t - Mid) / (t - t1) - y0;
             }

@@ -71,13 +71,18 @@ typedef struct {
     int mu_form;               /* 0: the EOS's own closed-form mu; 1: the PR expression of
                                   FluidCuda.cu:453 for every EOS, as the reference kernel does */
     int lap_points;            /* 5: GradD5 (PTCALCFIVEPOINT), 7: GradD7 (PTCALCSEVENPOINT) */
-    int wall_scheme;           /* 1: layered (COMPLEXCHEMBOUNDARY), 0: SIMPLECHEMBOUNDARY */
+    int wall_scheme;           /* 1: layered (COMPLEXCHEMBOUNDARY), 0: SIMPLECHEMBOUNDARY,
+                                  2: ChemBoundaryComplex (FluidCuda.cu:403-429): solid layer d
+                                  (LEVELd of Common.h:40-42, d = distance to the fluid) is the
+                                  D3Q19-weighted average of its neighbours in layer d-1 */
     int form;                  /* CPMP_FORM_* */
     int device;                /* HIP device ordinal */
     double A, B, T;            /* EOS constants (SetMultiphase; T = Tr*Tc) */
     double K, Ka, K1, K2;      /* EOS scale, kappa, surface-tension tuning (FluidCuda.cu:124-127) */
     double Ts[19];             /* MRT relaxation rates (FluidCuda.cu:154-157) */
     double rho0;               /* not used: the FP16 form takes its shift density per node */
+    int forcing;               /* 0: exact-difference force term (FluidCuda.cu:326-341),
+                                  1: Guo forcing in MRT form (no adjoint) */
 } cpmp_params;
 
 /* Generic initial condition: up to 8 spheres with tanh interfaces, optional
@@ -268,6 +273,10 @@ const char *cpmp_version(void);
    term, u3 = j(f)/rho used by it.  qp may be NULL (K1 = K2 = 0). */
 int cpmp_selftest_collide(const double *f19, double rho, const double *F3, const double *ts19,
                           const double *qp19, double *out19, double *u3);
+/* The same with the Guo force term of cpmp_params.forcing = 1 instead of the
+   exact-difference term; u3 = j(f)/rho. */
+int cpmp_selftest_collide_guo(const double *f19, double rho, const double *F3, const double *ts19,
+                              const double *qp19, double *out19, double *u3);
 
 #ifdef __cplusplus
 }

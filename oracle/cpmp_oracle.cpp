@@ -31,6 +31,13 @@
  *     non-zero; with K1=K2=0 it is a signed zero and leaves every moment unchanged.
  *   - nwall_lo = nwall_hi = 0 gives a fully periodic box (the reference always
  *     has 3+3 layers).
+ *   - wall_scheme 2 runs ChemBoundaryComplex (commented out in the reference,
+ *     FluidCuda.cu:403-429, 1281-1283) on level tags the reference never assigns:
+ *     a solid node's level is its distance in layers to the fluid (LEVEL1..3 of
+ *     Common.h:40-42 for the reference's three layers, deeper walls continue).
+ *   - forcing 1 replaces the exact-difference force term by Guo's (Guo, Zheng and
+ *     Shi 2002) in MRT form: equilibrium velocity (j + F/2)/rho, source
+ *     (I - S/2) M Fbar with Fbar_f = w_f [3 (e_f - u) + 9 (e_f.u) e_f].F.
  *
  * Pinning (tests/test_oracle_*.py):
  *   tables      M rows orthogonal with the row norms of the d'Humieres basis, R*M = I
@@ -40,7 +47,11 @@
  *   wall fill   constant and in-plane-linear fields reproduced; brute-force matrix form
  *   collide     mass/momentum identities, rest state fixed point, dense-matrix brute force
  *   reduce      exact sums on integer-valued fields
- *   contact ang synthetic spherical caps with known angle
+ *   contact ang wedges and caps of known angle, height and base (exact), tanh-profiled
+ *               spherical caps within the method's discretisation bound
+ *   init        SetFlowField against a NumPy restatement with the float32 radius
+ *   noise       published SplitMix64 output sequence; uniformity of the [-1,1) map
+ *   Guo         mass exact, momentum + F, agreement with the exact-difference force to O(F^2)
  *   adjoint     central finite differences of orc_step
  */
 #include "cpmp_oracle.h"
@@ -270,7 +281,29 @@ void stage_wall(orc_state *s) {
     const int lo = p.nwall_lo, hi = p.nwall_hi, nz = p.nz;
     if (lo == 0 && hi == 0) return;
     const int nl = std::max(lo, hi);
-    for (int pass = 0; pass < nl; ++pass) {
+    if (p.wall_scheme == 2) {
+        /* ChemBoundaryComplex(Type, Dens, LEVELd, LEVEL(d-1)), d = 1, 2, ...: the
+           average over f = 1..18 of the neighbours tagged d-1, weights Alpha */
+        auto level = [&](int k) { return k < lo ? lo - k : (k > nz - 1 - hi ? k - (nz - hi) + 1 : 0); };
+        for (int d = 1; d <= nl; ++d) {
+            #pragma omp parallel for collapse(2) schedule(static)
+            for (int i = 0; i < p.nx; ++i)
+                for (int j = 0; j < p.ny; ++j)
+                    for (int k = 0; k < nz; ++k) {
+                        if (level(k) != d) continue;
+                        double avg_den = 0, w = 0;
+                        for (int f = 1; f < Q; ++f) {
+                            const int ii = wrap(i + EX[f], p.nx), jj = wrap(j + EY[f], p.ny), kk = wrap(k + EZ[f], nz);
+                            if (level(kk) == d - 1) {
+                                avg_den += TB.w[f] * rho[s->I(ii, jj, kk)];
+                                w += TB.w[f];
+                            }
+                        }
+                        rho[s->I(i, j, k)] = avg_den / w;
+                    }
+        }
+    }
+    for (int pass = 0; pass < (p.wall_scheme == 2 ? 0 : nl); ++pass) {
         #pragma omp parallel for collapse(2) schedule(static)
         for (int i = 0; i < p.nx; ++i)
             for (int j = 0; j < p.ny; ++j) {
@@ -335,16 +368,19 @@ void stage_collide(orc_state *s) {
     const double *rho = s->rho.data();
     double *dist = s->f.data(), *temp = s->ftmp.data();
     const bool qp_on = (p.K1 != 0.0 || p.K2 != 0.0);
+    const bool guo = p.forcing == 1;
     #pragma omp parallel for collapse(2) schedule(static)
     for (int i = 0; i < p.nx; ++i)
         for (int j = 0; j < p.ny; ++j)
             for (int k = 0; k < p.nz; ++k) {
                 int64_t I = s->I(i, j, k);
                 if (s->type[I] != FLUID) continue;
-                const double Vx = s->vx[I], Vy = s->vy[I], Vz = s->vz[I];
-                s->mvx[I] = Vx + s->fx[I] / rho[I] * 0.5;
-                s->mvy[I] = Vy + s->fy[I] / rho[I] * 0.5;
-                s->mvz[I] = Vz + s->fz[I] / rho[I] * 0.5;
+                const double Vx0 = s->vx[I], Vy0 = s->vy[I], Vz0 = s->vz[I];
+                s->mvx[I] = Vx0 + s->fx[I] / rho[I] * 0.5;
+                s->mvy[I] = Vy0 + s->fy[I] / rho[I] * 0.5;
+                s->mvz[I] = Vz0 + s->fz[I] / rho[I] * 0.5;
+                /* Guo: the equilibrium velocity carries half the force */
+                const double Vx = guo ? s->mvx[I] : Vx0, Vy = guo ? s->mvy[I] : Vy0, Vz = guo ? s->mvz[I] : Vz0;
                 double EVx = Vx + s->fx[I] / rho[I];
                 double EVy = Vy + s->fy[I] / rho[I];
                 double EVz = Vz + s->fz[I] / rho[I];
@@ -385,6 +421,21 @@ void stage_collide(orc_state *s) {
                 }
                 for (int f = 0; f < Q; ++f)
                     Mf[f] = Mf[f] - p.Ts[f] * (Mf[f] - Meq[f]) + Qp[f] * p.Ts[f];
+                if (guo) {
+                    /* (I - S/2) M Fbar */
+                    const double Fx = s->fx[I], Fy = s->fy[I], Fz = s->fz[I];
+                    double Fb[Q];
+                    for (int q = 0; q < Q; ++q) {
+                        const double eu = EX[q] * Vx + EY[q] * Vy + EZ[q] * Vz;
+                        Fb[q] = TB.w[q] * (3 * ((EX[q] - Vx) * Fx + (EY[q] - Vy) * Fy + (EZ[q] - Vz) * Fz)
+                                           + 9 * eu * (EX[q] * Fx + EY[q] * Fy + EZ[q] * Fz));
+                    }
+                    for (int f = 0; f < Q; ++f) {
+                        double G = 0;
+                        for (int q = 0; q < Q; ++q) G += TB.M[f][q] * Fb[q];
+                        Mf[f] += (1 - p.Ts[f] / 2) * G;
+                    }
+                }
                 double post[Q];
                 for (int f = 0; f < Q; ++f) {
                     post[f] = 0;
@@ -393,7 +444,7 @@ void stage_collide(orc_state *s) {
                 /* EDM force + push streaming with half-way bounce-back */
                 for (int f = 0; f < Q; ++f) {
                     int ii = s->sx[3 + EX[f]][i], jj = s->sy[3 + EY[f]][j], kk = s->sz[3 + EZ[f]][k];
-                    double Df = post[f] + feq(f, Den, EVx, EVy, EVz) - feq(f, Den, Vx, Vy, Vz);
+                    double Df = guo ? post[f] : post[f] + feq(f, Den, EVx, EVy, EVz) - feq(f, Den, Vx, Vy, Vz);
                     int64_t T = s->I(ii, jj, kk);
                     if (s->type[T] == FLUID) temp[f * N + T] = Df;
                     else temp[(int64_t)TB.re[f] * N + I] = Df;
@@ -644,13 +695,17 @@ int orc_reduce(orc_state *s, double *out) {
 }
 
 int orc_contact_angle(const orc_params *p, const double *rho, double den_g, double den_l, int tier, double *out) {
-    /* ContactAngle(side=0, tier), FluidCuda.cu:557-689, on the XZ plane y=tier */
+    /* ContactAngle(side=0, tier), FluidCuda.cu:557-689, on the XZ plane y=tier.  The
+       reference scans rows z = 2..5 and extrapolates to z = 2, the last of its three
+       solid layers; here the rows are offset by the bottom wall depth (begin = lo - 1),
+       which is the reference for lo = 3. */
     const int DX = p->nx, DY = p->ny, DZ = p->nz;
+    if (p->nwall_lo < 1 || p->nwall_lo + 4 >= DZ) return -1;
     const double PI2 = 3.14159265358979323846264338327950288;
     const double Mid = (den_g + den_l) / 2;
     auto T = [&](int i, int k) { return rho[((int64_t)i * DY + tier) * DZ + k]; };
     double Px[8] = {0};
-    const int begin = 2;
+    const int begin = p->nwall_lo - 1;
     for (int j = begin; j <= begin + 3; ++j) {
         int n = (j - begin) * 2;
         for (int i = 0; i < DX / 2 + 2; ++i) {
@@ -662,7 +717,7 @@ int orc_contact_angle(const orc_params *p, const double *rho, double den_g, doub
             if (t1 < Mid && t2 >= Mid) { Px[n + 1] = double(i - 1) - (t1 - Mid) / (t1 - t2); break; }
         }
     }
-    double y0 = 2, y1 = 3, y2 = 4;
+    double y0 = begin, y1 = begin + 1, y2 = begin + 2;
     double CaLeft, CaRight, CaCap = 0, XLeft, XRight, H = NAN, L = 0;
     int valid = 0;
     if (Px[0] != 0.0 && Px[1] != 0.0) {
@@ -676,7 +731,7 @@ int orc_contact_angle(const orc_params *p, const double *rho, double den_g, doub
         if (Ca < 0) Ca += PI2;
         CaRight = Ca / PI2 * 180;
         L = XRight - XLeft;
-        for (int i = int((Px[4] + Px[5]) / 2 + 0.5), j = 2; j < DZ - 2; ++j) {
+        for (int i = int((Px[4] + Px[5]) / 2 + 0.5), j = begin; j < DZ - 2; ++j) {
             double t = T(i, j), t1 = T(i, j + 1);
             if (t >= Mid && t1 < Mid) H = double(j) + (t - Mid) / (t - t1) - y0;
         }
@@ -693,6 +748,8 @@ int orc_contact_angle(const orc_params *p, const double *rho, double den_g, doub
     out[7] = valid;
     return 0;
 }
+
+uint64_t orc_splitmix64(uint64_t x) { return splitmix64(x); }
 
 double orc_noise(uint64_t seed, uint64_t index) {
     uint64_t r = splitmix64(seed ^ splitmix64(index));
@@ -728,6 +785,7 @@ void fwd_record(orc_state *s, Tape &t) {
 extern "C" int orc_adjoint(orc_state *s, int nsteps, const double *target, double *J,
                            double *g_mu_lo, double *g_mu_hi, double *g_ka) {
     const orc_params &p = s->p;
+    if (p.forcing != 0 || p.wall_scheme == 2) return -1;
     const int64_t N = s->N;
     const int64_t NXY = (int64_t)p.nx * p.ny;
     std::vector<Tape> tape(nsteps + 1);

@@ -32,10 +32,13 @@ typedef struct {
     int mu_form;             /* 0: each EOS's own closed-form mu; 1: the PR expression of
                                 FluidCuda.cu:453 for every EOS (what the reference kernel does) */
     int lap_points;          /* 5: GradD5 (PTCALCFIVEPOINT), 7: GradD7 (PTCALCSEVENPOINT) */
-    int wall_scheme;         /* 1: COMPLEXCHEMBOUNDARY (layered), 0: SIMPLECHEMBOUNDARY */
+    int wall_scheme;         /* 1: COMPLEXCHEMBOUNDARY (layered), 0: SIMPLECHEMBOUNDARY,
+                                2: ChemBoundaryComplex (FluidCuda.cu:403-429, levels of
+                                Common.h:40-42) */
     double A, B, T;          /* EOS constants after SetMultiphase (FluidCuda.cu:1321-1353) */
     double K, Ka, K1, K2;    /* FluidCuda.cu:124-127 */
     double Ts[19];           /* relaxation rates (FluidCuda.cu:154-157) */
+    int forcing;             /* 0: exact-difference term (FluidCuda.cu:326-341), 1: Guo */
 } orc_params;
 
 typedef struct orc_state orc_state;
@@ -73,18 +76,23 @@ int    orc_stage(orc_state *s, int stage);   /* 0 wall, 1 mu, 2 force, 3 collide
 /* CalcMacroCPU (FluidCuda.cu:1626-1657): out[0]=mass, [1]=nan/neg count, [2]=max speed */
 int    orc_reduce(orc_state *s, double *out);
 
-/* ContactAngle(side=0, tier) (FluidCuda.cu:557-689) on rho of the state.
-   out: CaLeft, CaRight, CaCap, XLeft, XRight, H, L, valid */
+/* ContactAngle(side=0, tier) (FluidCuda.cu:557-689) on rho of the state, rows
+   offset by the bottom wall depth (the reference's rows for nwall_lo = 3); -1 when
+   nwall_lo < 1.  out: CaLeft, CaRight, CaCap, XLeft, XRight, H, L, valid */
 int    orc_contact_angle(const orc_params *p, const double *rho, double den_g, double den_l,
                          int tier, double *out);
 
+/* SplitMix64 output function: the generator's next value for state x, i.e. the mix
+   of x + 0x9E3779B97F4A7C15 (Steele, Lea, Flood 2014) */
+uint64_t orc_splitmix64(uint64_t x);
 /* SplitMix64-based uniform noise in [-1,1) keyed by (seed, index) */
 double orc_noise(uint64_t seed, uint64_t index);
 
 /* ---- discrete adjoint (reverse mode of orc_step) ---- */
 /* J = 1/2 sum_fluid (rho^N - target)^2 after nsteps from the state's current contents.
    g_mu_lo: dJ/d mu_w(bottom) [nx*ny], g_mu_hi: dJ/d mu_w(top) [nx*ny], g_ka: dJ/dKa.
-   The forward trajectory is stored in memory (oracle sizes are small).  */
+   The forward trajectory is stored in memory (oracle sizes are small).  -1 for the
+   Guo force term and for wall scheme 2, which it does not cover. */
 int    orc_adjoint(orc_state *s, int nsteps, const double *target, double *J,
                    double *g_mu_lo, double *g_mu_hi, double *g_ka);
 

@@ -55,6 +55,25 @@ CASES = {
     # asymmetric wall layers and a top wall field of its own
     "prm_walls_2_4": dict(dims=(20, 18, 30), eos="PRM", tr=0.70, nwall=2, nwall_hi=4, init="reference",
                           radius=6, width=5, mu_w=-0.01, mu_w_hi=0.02, steps=100),
+    # Redlich-Kwong-Soave (FluidCuda.cu:1333-1336) with its own mu, walls 3 + 3
+    "rks_walls": dict(dims=(28, 20, 28), eos="RKS", tr=0.70, nwall=3, init="reference",
+                      radius=8, width=5, mu_w=-0.01, steps=150),
+    # van der Waals (FluidCuda.cu:1320-1323) with its own mu (the large striped-wall
+    # configuration's EOS), walls 3 + 3
+    "vdw_own_mu": dict(dims=(28, 20, 28), eos="VDW", tr=0.80, nwall=3, init="reference",
+                       radius=8, width=5, mu_w=0.02, steps=150),
+    # Guo forcing instead of the exact-difference term, walls 3 + 3, K1 on (van der
+    # Waals, density ratio 8: Guo's term diverges within 20 steps at the ratios of
+    # 145 and 258 of the PR and RK cases above, where the exact-difference term is stable)
+    "guo_vdw_walls": dict(dims=(30, 22, 26), eos="VDW", tr=0.80, nwall=3, init="reference",
+                          radius=8, width=5, mu_w=0.01, forcing=1, K1=0.003, steps=200),
+    # ChemBoundaryComplex (D3Q19-weighted level averages), walls 3 + 2, periodic droplets
+    "complex_wall": dict(dims=(26, 24, 30), eos="PRW", tr=0.70, nwall=3, nwall_hi=2, init="reference",
+                         radius=8, width=5, mu_w=-0.01, mu_w_hi=0.03, wall_scheme=2, steps=200),
+    # Guo forcing in a periodic box (the FP32 tiled geometry), Redlich-Kwong at Tr 0.85
+    # (density ratio 58)
+    "guo_rke_periodic": dict(dims=(32, 16, 32), eos="RKE", tr=0.85, nwall=0, init="spheres",
+                             spheres=[(16, 8, 16, 7)], width=5, noise=1e-4, forcing=1, steps=150),
 }
 
 
@@ -64,7 +83,7 @@ def build(case, cpmp, form="fp64"):
     kw = dict(eos=eos, tr=tr, nwall=case["nwall"], nwall_hi=case.get("nwall_hi"),
               mu_form=case.get("mu_form", 0), lap_points=case.get("lap_points", 5),
               wall_scheme=case.get("wall_scheme", 1), K1=case.get("K1", 0.0), K2=-case.get("K1", 0.0),
-              Ka=case.get("Ka", 0.001), tau=case.get("tau", 0.8))
+              Ka=case.get("Ka", 0.001), tau=case.get("tau", 0.8), forcing=case.get("forcing", 0))
     po = orc.make_params(nx, ny, nz, **kw)
     o = orc.Oracle(po)
     pg = cpmp.make_params(nx, ny, nz, form=form, **kw)

@@ -25,7 +25,7 @@ class OrcParams(C.Structure):
                 ("wall_scheme", C.c_int),
                 ("A", C.c_double), ("B", C.c_double), ("T", C.c_double),
                 ("K", C.c_double), ("Ka", C.c_double), ("K1", C.c_double), ("K2", C.c_double),
-                ("Ts", C.c_double * 19)]
+                ("Ts", C.c_double * 19), ("forcing", C.c_int)]
 
 
 _lib = None
@@ -61,7 +61,10 @@ def lib():
         L.orc_contact_angle.argtypes = [C.POINTER(OrcParams), p, d, d, i, p]
         L.orc_noise.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_noise.restype = d
+        L.orc_splitmix64.argtypes = [C.c_uint64]
+        L.orc_splitmix64.restype = C.c_uint64
         L.orc_adjoint.argtypes = [p, i, p, p, p, p, p]
+        L.orc_adjoint.restype = i
         _lib = L
     return _lib
 
@@ -88,7 +91,7 @@ def reference_ts(tau):
 
 
 def make_params(nx, ny, nz, eos="PRW", tr=0.7, tau=0.8, K=0.2, Ka=0.001, K1=0.0, K2=0.0,
-                nwall=3, nwall_hi=None, mu_form=0, lap_points=5, wall_scheme=1, ts=None):
+                nwall=3, nwall_hi=None, mu_form=0, lap_points=5, wall_scheme=1, ts=None, forcing=0):
     A, B, Tc, Rc = set_multiphase(eos, tr)
     p = OrcParams()
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -99,6 +102,7 @@ def make_params(nx, ny, nz, eos="PRW", tr=0.7, tau=0.8, K=0.2, Ka=0.001, K1=0.0,
     p.K, p.Ka, p.K1, p.K2 = K, Ka, K1, K2
     for q, v in enumerate(ts if ts is not None else reference_ts(tau)):
         p.Ts[q] = v
+    p.forcing = forcing
     return p
 
 
@@ -157,7 +161,8 @@ class Oracle:
         t = np.ascontiguousarray(target, dtype=np.float64)
         J = np.zeros(1)
         glo, ghi, gka = np.zeros(nxy), np.zeros(nxy), np.zeros(1)
-        lib().orc_adjoint(self.h, nsteps, _ptr(t), _ptr(J), _ptr(glo), _ptr(ghi), _ptr(gka))
+        if lib().orc_adjoint(self.h, nsteps, _ptr(t), _ptr(J), _ptr(glo), _ptr(ghi), _ptr(gka)) != 0:
+            raise ValueError("the oracle adjoint does not cover these parameters")
         return J[0], glo, ghi, gka[0]
 
 
@@ -184,10 +189,17 @@ def maxwell(eos, A, B, T, g0, l0):
 
 
 def contact_angle(params, rho, den_g, den_l, tier):
+    """CaLeft, CaRight, CaCap, XLeft, XRight, H, L, valid; None when the wall is
+    too thin for the scan"""
     out = np.zeros(8)
     a = np.ascontiguousarray(rho, dtype=np.float64)
-    lib().orc_contact_angle(C.byref(params), _ptr(a), den_g, den_l, tier, _ptr(out))
+    if lib().orc_contact_angle(C.byref(params), _ptr(a), den_g, den_l, tier, _ptr(out)) != 0:
+        return None
     return out
+
+
+def splitmix64(x):
+    return lib().orc_splitmix64(x)
 
 
 def noise(seed, index):

@@ -7,8 +7,10 @@ the step, as with the separate density pass.  Lattices that the 32 x 8 x 32
 (x, y, z) tiles cover exactly take the fused path: one tile or several in each
 direction (a single tile reads its own border ring through the periodic wrap),
 walls of equal and unequal thickness, periodic boxes.  The other geometries
-(partial tiles, short chunks, ny = 1, nx below the chunk length) take the
-separate density pass and are checked the same way."""
+(partial tiles, ny = 1, nx below the chunk length) and lattices too small for
+the fused path (single-plane chunks) take the separate density pass and are
+checked the same way.  Chunks of 32 planes, the length large lattices select,
+are checked in test_gpu_production.py."""
 import os
 
 import numpy as np
@@ -23,13 +25,13 @@ GEOMETRIES = {
     "prw_2d": None,
     "prm_walls_2_4": None,
     "vdw_variants": None,
-    # 33 x planes: chunks of 32 + 1; y and z tiles both partial; periodic
+    # 33 x planes (an odd count); y and z tiles both partial; periodic
     "rke_33x13x21": dict(dims=(33, 13, 21), eos="RKE", tr=0.60, nwall=0,
                          spheres=[(16, 6, 10, 5)], width=4.0),
     # nx below the chunk length, two z tiles, walls 3 + 3
     "prw_5x9x40": dict(dims=(5, 9, 40), eos="PRW", tr=0.70, nwall=3,
                        spheres=[(2, 4, 3, 6)], width=4.0),
-    # three chunks (32 + 32 + 6), y tiles 8 + 8 + 1, z tiles 32 + 13
+    # 70 x planes, y tiles 8 + 8 + 1, z tiles 32 + 13
     "cse_70x17x45": dict(dims=(70, 17, 45), eos="CSE", tr=0.85, nwall=0,
                          spheres=[(35, 8, 22, 9)], width=2.5, Ka=0.01),
     # one x plane; the lattice folds onto itself in x

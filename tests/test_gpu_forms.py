@@ -145,26 +145,89 @@ def test_contact_angle_matches_oracle(cpmp):
     assert np.allclose(got, b[:7], rtol=1e-8, atol=1e-8, equal_nan=True)
 
 
+@pytest.mark.parametrize("lo", [3, 2, 4])
+def test_contact_angle_on_pinned_fields_matches_oracle(cpmp, lo):
+    """the device measurement (the XZ plane extracted on the device) on the wedge
+    and spherical-cap fields whose angles the oracle tests pin in closed form, with
+    the scan rows following the bottom wall depth; no bottom wall is an error"""
+    import test_oracle_pins as pins
+    nx, ny, nz = 64, 3, 72
+    base = lo - 1
+    for theta in (30, 60, 90, 120, 150):
+        H = 12.6 * np.tan(np.radians(theta) / 2)
+        fields = [pins.plane_field(nx, ny, nz, pins.wedge(31.3, 12.6, theta, base + H, base,
+                                                          rows={base: (0.41, -0.23), base + 3: (-0.17, 0.31)}))]
+        x, z = np.meshgrid(np.arange(nx, dtype=float), np.arange(nz, dtype=float), indexing="ij")
+        r = np.sqrt((x - 32.3) ** 2 + (z - (base - 25 * np.cos(np.radians(theta)))) ** 2)
+        cap = pins.MID - (pins.DL - pins.DG) / 2 * np.tanh(2 * (r - 25) / 2.5)
+        fields.append(np.repeat(cap[:, None, :], ny, axis=1).ravel())
+        p = orc.make_params(nx, ny, nz, eos="PRW", tr=0.7, nwall=lo)
+        g = cpmp.Lattice(cpmp.make_params(nx, ny, nz, eos="PRW", tr=0.7, nwall=lo))
+        for rho in fields:
+            g.set("rho", rho)
+            a = g.contact_angle(1, pins.DG, pins.DL)
+            b = orc.contact_angle(p, rho, pins.DG, pins.DL, 1)
+            got = np.array([a[k] for k in ("CaLeft", "CaRight", "CaCap", "XLeft", "XRight", "H", "L", "valid")])
+            assert np.array_equal(got, b, equal_nan=True), (theta, got, b)
+        g.close()
+    g = cpmp.Lattice(cpmp.make_params(nx, ny, nz, eos="PRW", tr=0.7, nwall=0))
+    with pytest.raises(cpmp.CPMPError):
+        g.contact_angle(1, pins.DG, pins.DL)
+    g.close()
+
+
+def _plt_zone_values(raw, n):
+    """the Density values (last variable, FP64) of a one-zone PLT of n nodes"""
+    return np.frombuffer(raw[-8 * n:], np.float64)
+
+
 def test_write_plt_layout(cpmp, tmp_path):
+    """full field: header magic and the density block in reference order; the
+    decimated forms keep every 5th node per axis, or (SaveTask's CompressField,
+    FluidCuda.cu:466-472) the linear index times 5"""
     case = gc.CASES["vdw_variants"]
     _, g = gc.build(case, cpmp)
     g.step(5)
-    path = str(tmp_path / "FlowField_5.plt")
+    nx, ny, nz = case["dims"]
+    rho = g.get("rho")
+    path = str(tmp_path / "full.plt")
     g.write_plt(path)
     raw = open(path, "rb").read()
     assert raw[:8] == b"#!TDV112"
-    nx, ny, nz = case["dims"]
-    n = nx * ny * nz
-    rho = g.get("rho")
-    assert np.array_equal(np.frombuffer(raw[-8 * n:], np.float64), rho)
-    # reference writer, when built from the reference sources (oracle/_ref)
+    assert np.array_equal(_plt_zone_values(raw, nx * ny * nz), rho)
+    lx, ly, lz = nx // 5, ny // 5, nz // 5
+    i, j, k = np.meshgrid(np.arange(lx), np.arange(ly), np.arange(lz), indexing="ij")
+    for linear, src in ((False, (i * 5 * ny + j * 5) * nz + k * 5), (True, ((i * ny + j) * nz + k) * 5)):
+        path = str(tmp_path / f"dec{int(linear)}.plt")
+        g.write_plt(path, decimate=5, decimate_linear=linear)
+        raw = open(path, "rb").read()
+        assert np.array_equal(_plt_zone_values(raw, lx * ly * lz), rho[src.ravel()])
+    g.close()
+
+
+@pytest.mark.parametrize("decimate", [1, 5])
+def test_write_plt_matches_reference_writer(cpmp, tmp_path, decimate):
+    """byte for byte against the reference's own ordered_tec writer (oracle/_ref/plt_ref,
+    built from the reference sources by oracle/ref.mk) driven as SaveTask drives it
+    (FluidCuda.cu:901-927): the full field and SaveTask's DR = 5 linear-stride
+    compression (FluidCuda.cu:466-472)"""
     ref_bin = os.path.join(os.path.dirname(orc.LIB_PATH), "..", "_ref", "plt_ref")
-    if os.path.exists(ref_bin):
-        import subprocess
-        rho_path = str(tmp_path / "rho.bin")
-        rho.tofile(rho_path)
-        subprocess.check_call([ref_bin, rho_path, str(nx), str(ny), str(nz), "1", str(tmp_path), "FlowField_5"])
-        assert open(str(tmp_path / "FlowField_5.plt"), "rb").read() == raw
+    if not os.path.exists(ref_bin):
+        pytest.skip("oracle/_ref/plt_ref (the reference's Tecplot writer) is not built")
+    import subprocess
+    case = gc.CASES["vdw_variants"]
+    _, g = gc.build(case, cpmp)
+    g.step(5)
+    nx, ny, nz = case["dims"]
+    name = f"FlowField_{decimate}"
+    ours = tmp_path / "ours"
+    ours.mkdir()
+    g.write_plt(str(ours / f"{name}.plt"), decimate=decimate, decimate_linear=decimate > 1)
+    rho_path = str(tmp_path / "rho.bin")
+    g.get("rho").tofile(rho_path)
+    subprocess.check_call([ref_bin, rho_path, str(nx), str(ny), str(nz), str(decimate), str(tmp_path), name])
+    assert open(str(tmp_path / f"{name}.plt"), "rb").read() == open(str(ours / f"{name}.plt"), "rb").read()
+    g.close()
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32"])

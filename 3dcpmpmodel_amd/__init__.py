@@ -28,7 +28,7 @@ EXPORTS = [
     "cpmp_coexistence", "cpmp_cocurve_lookup", "cpmp_create", "cpmp_destroy", "cpmp_get_params",
     "cpmp_memory", "cpmp_init_reference", "cpmp_init_splash", "cpmp_init_spheres", "cpmp_init_from_rho",
     "cpmp_set_wall_mu", "cpmp_step", "cpmp_sync", "cpmp_step_count", "cpmp_get_field",
-    "cpmp_set_field", "cpmp_reduce", "cpmp_contact_angle", "cpmp_cap_fit", "cpmp_probe",
+    "cpmp_set_field", "cpmp_reduce", "cpmp_reduce_noalloc", "cpmp_contact_angle", "cpmp_cap_fit", "cpmp_probe",
     "cpmp_laplace_jump",
     "cpmp_checkpoint_save", "cpmp_checkpoint_load", "cpmp_write_plt", "cpmp_adjoint", "cpmp_adjoint_plan", "cpmp_neck_radius",
     "cpmp_kernel_times", "cpmp_time_steps", "cpmp_algorithmic_bytes_per_node",
@@ -112,6 +112,7 @@ def _load():
     L.cpmp_get_field.argtypes = [p, i, p]
     L.cpmp_set_field.argtypes = [p, i, p]
     L.cpmp_reduce.argtypes = [p, C.POINTER(Stats)]
+    L.cpmp_reduce_noalloc.argtypes = [p, C.POINTER(Stats)]
     L.cpmp_contact_angle.argtypes = [p, i, d, d, p]
     L.cpmp_cap_fit.argtypes = [p, d, d, p]
     L.cpmp_checkpoint_save.argtypes = [p, C.c_char_p]

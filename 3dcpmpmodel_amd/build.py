@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libcpmp.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "api.hip", "adjoint.hip", "observables.cpp", "plt.cpp"]
+SOURCES = ["kernels.hip", "api.hip", "adjoint.hip", "observables.cpp", "plt.cpp", "noalloc.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -516,6 +516,12 @@ int cpmp_reduce(cpmp_ctx *c, cpmp_stats *out) {
     HIPCHK(hipSetDevice(c->p.device));
     double v[12];
     HIPCHK(launch_reduce(c, v));
+    return stats_from_reduction(c, v, out);
+}
+
+}  // extern "C"
+
+int cpmp::stats_from_reduction(cpmp_ctx *c, const double *v, cpmp_stats *out) {
     out->mass = v[0];
     out->bad_count = (int64_t)v[1];
     out->first_bad = v[1] > 0 ? (int64_t)v[2] : -1;
@@ -532,6 +538,8 @@ int cpmp_reduce(cpmp_ctx *c, cpmp_stats *out) {
     }
     return CPMP_OK;
 }
+
+extern "C" {
 
 // ---------------------------------------------------------------------------
 // Checkpoint: magic, version, params, step, cur populations in storage form (in

@@ -117,6 +117,11 @@ hipError_t launch_cap_columns(cpmp_ctx *c, double mid, double *part, unsigned *n
 hipError_t launch_plane_fraction(cpmp_ctx *c, int x, double rg, double rl, double *part, unsigned *nblocks);
 hipError_t launch_force(cpmp_ctx *c, double *out3);
 hipError_t launch_reduce(cpmp_ctx *c, double *host_out /* 12 */);
+// the same reduction with its partials (reduce_scratch_doubles() values) in `part`
+hipError_t launch_reduce_into(cpmp_ctx *c, double *part, double *host_out /* 12 */);
+// cpmp_stats from the 12 reduction values; CPMP_ENAN (naming the first bad node) when
+// a density is NaN or negative
+int stats_from_reduction(cpmp_ctx *c, const double *v, cpmp_stats *out);
 hipError_t launch_type(cpmp_ctx *c, double *out);
 size_t reduce_scratch_doubles();
 

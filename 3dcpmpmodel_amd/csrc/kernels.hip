@@ -1758,21 +1758,23 @@ hipError_t launch_type(cpmp_ctx *c, double *out) {
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(cpmp_ctx *c, double *host_out) {
+hipError_t launch_reduce_into(cpmp_ctx *c, double *part, double *host_out) {
     hipError_t e = dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
         hipLaunchKernelGGL(k_reduce<TS>, dim3(RED_BLOCKS), dim3(TPB), 0, c->stream, c->dp, (const TS *)c->rho, c->mv,
-                           c->mv_valid ? 1 : 0, c->red);
+                           c->mv_valid ? 1 : 0, part);
         return hipGetLastError();
     });
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(TPB), 0, c->stream, c->red, RED_BLOCKS, c->red + RED_BLOCKS * RED_VALS);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(TPB), 0, c->stream, part, RED_BLOCKS, part + RED_BLOCKS * RED_VALS);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(host_out, c->red + RED_BLOCKS * RED_VALS, RED_VALS * sizeof(double), hipMemcpyDeviceToHost,
+    if ((e = hipMemcpyAsync(host_out, part + RED_BLOCKS * RED_VALS, RED_VALS * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream)) != hipSuccess)
         return e;
     return hipStreamSynchronize(c->stream);
 }
+
+hipError_t launch_reduce(cpmp_ctx *c, double *host_out) { return launch_reduce_into(c, c->red, host_out); }
 
 size_t reduce_scratch_doubles() { return (size_t)(RED_BLOCKS + 1) * RED_VALS; }
 

@@ -207,6 +207,14 @@ int cpmp_set_field(cpmp_ctx *ctx, int field, const double *host);   /* RHO, MU o
 
 /* ---- diagnostics ---- */
 int cpmp_reduce(cpmp_ctx *ctx, cpmp_stats *out);
+/* cpmp_reduce without context scratch (no counterpart in the reference): for callers
+   that cannot allocate device memory around the call (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  The same kernels and
+   results as cpmp_reduce, bit for bit; the reduction partials live in a static
+   device array of 513 x 12 doubles (49 248 bytes) per GPU, held until the process
+   ends, which the fixed 512-block grid fits for any lattice.  The call allocates
+   nothing.  Calls that overlap on one GPU share that array and race. */
+int cpmp_reduce_noalloc(cpmp_ctx *ctx, cpmp_stats *out);
 /* ContactAngle(side=0, tier) (FluidCuda.cu:557-689) on the XZ plane y=tier, with the
    interface density (rho_g+rho_l)/2.  out[8]: CaLeft, CaRight, CaCap (degrees),
    XLeft, XRight, H, L, valid. */

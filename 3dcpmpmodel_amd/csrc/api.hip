@@ -282,7 +282,9 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
     if (hipMalloc(&c->mu, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "mu");
     const char *fuse = std::getenv("CPMP_FUSE");
     size_t ring_elems = 0, chunk_elems = 0;
-    if (c->form != CPMP_FORM_FP64 && !(fuse && std::strcmp(fuse, "0") == 0) &&
+    // every storage form sums the density inside the march collide where the tiles
+    // cover the lattice (not with the one-thread-per-node collide of small FP64 lattices)
+    if (!c->collide_v1 && !(fuse && std::strcmp(fuse, "0") == 0) &&
         fuse_geometry(p->nx, p->ny, p->nz, c->dp.xc, &ring_elems, &chunk_elems)) {
         if (hipMalloc(&c->rho_next, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "next rho");
         if (hipMalloc(&c->rbuf, ring_elems * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "tile rings");

@@ -18,7 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcpmp.so")
 
 EOS = {"VDW": 0, "CSE": 1, "RKE": 2, "RKS": 3, "PRW": 4, "PRM": 5}
-FORM = {"fp64": 0, "fp32": 1, "fp16": 2}
+FORM = {"fp64": 0, "fp32": 1, "fp16": 2, "fp32a": 3}
 FIELD = {"type": 0, "rho": 1, "mu": 2, "f": 3, "vel": 4, "mv": 5, "force": 6}
 Q = 19
 

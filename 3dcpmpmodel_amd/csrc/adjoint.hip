@@ -231,7 +231,7 @@ __device__ __forceinline__ void adj_stencil_march(const DevParams &P, const doub
         off[r] = (long long)wrapc(y0 - kAdjHalo + ii / kAdjEZ, P.ny) * P.nz + wrapc(z0 - kAdjHalo + ii % kAdjEZ, P.nz);
     }
     double pre[kAdjSlots][3];
-    auto fetch = [&](int xp) {
+    auto fetch = [&](int xp) {   // wraps past the end (loads unused then)
         const double *src = A + (long long)wrapc(xp, P.nx) * plane;
 #pragma unroll
         for (int r = 0; r < kAdjSlots; ++r)
@@ -242,17 +242,20 @@ __device__ __forceinline__ void adj_stencil_march(const DevParams &P, const doub
 #pragma unroll
     for (int k = 0; k < 7; ++k) acc[k] = 0;
     const int c0 = (ty + kAdjHalo) * kAdjEZ + tz + kAdjHalo;
-    const double *ax = sf[0], *ay = sf[1], *az = sf[2];
+    // the planes alternate between two LDS buffers: a wave stores plane xp + 1 while
+    // slower waves still read plane xp, so one barrier per plane suffices
+    int buf = 0;
     fetch(xa - kAdjHalo);
     for (int xp = xa - kAdjHalo; xp <= xb + kAdjHalo - 1; ++xp) {
-        __syncthreads();   // the previous plane is no longer read
+        double(*sb)[kAdjEA] = sf + 3 * buf;
 #pragma unroll
         for (int r = 0; r < kAdjSlots; ++r)
             if (pos[r] >= 0)
 #pragma unroll
-                for (int f = 0; f < 3; ++f) sf[f][pos[r]] = pre[r][f];
+                for (int f = 0; f < 3; ++f) sb[f][pos[r]] = pre[r][f];
         __syncthreads();
-        fetch(xp + 1);     // next plane in flight during the arithmetic (wraps past the end)
+        fetch(xp + 1);     // next plane in flight during the arithmetic
+        const double *ax = sb[0], *ay = sb[1], *az = sb[2];
         const double x0 = 2 * ax[c0];
         double inpl = 0;
 #pragma unroll
@@ -280,6 +283,7 @@ __device__ __forceinline__ void adj_stencil_march(const DevParams &P, const doub
 #pragma unroll
         for (int k = 0; k < 6; ++k) acc[k] = acc[k + 1];
         acc[6] = 0;
+        buf ^= 1;
     }
 }
 
@@ -291,7 +295,7 @@ __global__ void __launch_bounds__(TPB) k_adj_stencil_march(const DevParams P, co
                                                            const double *__restrict__ rb_loc, double *__restrict__ mub,
                                                            double *__restrict__ rb_new, int ntz, int nty, int xc,
                                                            int nb) {
-    __shared__ double sf[3][kAdjEA];   // x, y, z components of one plane
+    __shared__ double sf[2 * 3][kAdjEA];   // x, y, z components of two planes
     const int b = blockIdx.x;
     if (b < nb) adj_stencil_march<false>(P, Pf, nullptr, mub, ntz, nty, xc, b, sf);
     else adj_stencil_march<true>(P, Wf, rb_loc, rb_new, ntz, nty, xc, b - nb, sf);

@@ -50,8 +50,8 @@ hipError_t sync_reference_density(cpmp_ctx *c) {
     return hipMemcpyAsync(c->rho_next, c->rho, c->N * c->scal_bytes, hipMemcpyDeviceToDevice, c->stream);
 }
 
-int pop_size(int form) { return form == CPMP_FORM_FP64 ? 8 : (form == CPMP_FORM_FP32 ? 4 : 2); }
-int scal_size(int form) { return form == CPMP_FORM_FP64 ? 8 : 4; }
+int pop_size(int form) { return fp64_storage(form) ? 8 : (form == CPMP_FORM_FP32 ? 4 : 2); }
+int scal_size(int form) { return fp64_storage(form) ? 8 : 4; }
 
 // interface runs of the chemical-potential march (FP32 storage forms): 64 * xc
 // slots per wave, MT / 64 waves per workgroup
@@ -103,7 +103,7 @@ int check_params(const cpmp_params *p) {
         return set_error(CPMP_EINVAL, "fewer than 4 fluid layers between the walls");
     if (p->eos < 0 || p->eos > 5) return set_error(CPMP_EINVAL, "eos id %d", p->eos);
     if (p->lap_points != 5 && p->lap_points != 7) return set_error(CPMP_EINVAL, "lap_points must be 5 or 7");
-    if (p->form < 0 || p->form > 2) return set_error(CPMP_EINVAL, "form %d", p->form);
+    if (p->form < 0 || p->form > 3) return set_error(CPMP_EINVAL, "form %d", p->form);
     if (p->wall_scheme < 0 || p->wall_scheme > 2) return set_error(CPMP_EINVAL, "wall_scheme %d", p->wall_scheme);
     if (p->forcing != 0 && p->forcing != 1) return set_error(CPMP_EINVAL, "forcing %d", p->forcing);
     if (!(p->B > 0) || !(p->T > 0)) return set_error(CPMP_EINVAL, "EOS constants B=%g T=%g", p->B, p->T);
@@ -301,7 +301,7 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
         if (hipMalloc(&c->rho_next, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "previous rho");
         c->fuse_bytes = (size_t)c->N * c->scal_bytes;
     }
-    if (c->form != CPMP_FORM_FP64 && hipMalloc((void **)&c->band, band_slots(c) * sizeof(uint4)) != hipSuccess)
+    if (!fp64_storage(c->form) && hipMalloc((void **)&c->band, band_slots(c) * sizeof(uint4)) != hipSuccess)
         return fail(CPMP_ENOMEM, "interface runs");
     if (hipMalloc((void **)&c->mv, 3 * c->N * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "MV");
     if (hipMalloc((void **)&c->mu_w, 2 * c->NXY * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "wall mu");
@@ -649,6 +649,9 @@ int cpmp_kernel_times(cpmp_ctx *c, int nsteps, double *ms, int *launches) {
     hipEvent_t ev[8];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     double acc[CPMP_K_COUNT] = {0, 0, 0, 0};
+    // the last step's collide also writes the MV field (a slower instantiation): the
+    // collide average covers the steps before it, unless there is only one
+    const int ncol = nsteps > 1 ? nsteps - 1 : 1;
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int n = 0; n < nsteps; ++n) {
         HIPCHK(hipEventRecord(ev[0], c->stream));
@@ -665,6 +668,7 @@ int cpmp_kernel_times(cpmp_ctx *c, int nsteps, double *ms, int *launches) {
         HIPCHK(hipEventRecord(ev[7], c->stream));
         HIPCHK(hipEventSynchronize(ev[7]));
         for (int k = 0; k < CPMP_K_COUNT; ++k) {
+            if (k == CPMP_K_COLLIDE && n >= ncol) continue;
             float t = 0;
             HIPCHK(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
             acc[k] += t;
@@ -674,7 +678,7 @@ int cpmp_kernel_times(cpmp_ctx *c, int nsteps, double *ms, int *launches) {
     for (auto &e : ev) hipEventDestroy(e);
     const int per[CPMP_K_COUNT] = {passes, 1, 1, 1};
     for (int k = 0; k < CPMP_K_COUNT; ++k) {
-        ms[k] = per[k] ? acc[k] / nsteps / per[k] : 0.0;
+        ms[k] = per[k] ? acc[k] / (k == CPMP_K_COLLIDE ? ncol : nsteps) / per[k] : 0.0;
         if (launches) launches[k] = per[k];
     }
     c->mv_valid = true;

@@ -83,6 +83,8 @@ hipError_t launch_wall(cpmp_ctx *c);
 hipError_t launch_mu(cpmp_ctx *c, void *rho_copy = nullptr);
 bool density_fused(const cpmp_ctx *c);
 bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *chunk_elems);
+// forms whose populations, rho and mu are stored in FP64 (FP64, FP32A)
+inline bool fp64_storage(int form) { return form == CPMP_FORM_FP64 || form == CPMP_FORM_FP32A; }
 // x planes per march workgroup for a lattice and storage form (CPMP_XC overrides)
 int march_chunk(int nx, int ny, int nz, int form);
 // the one-thread-per-node collide instead of the x march (CPMP_COLLIDE=v1 / march

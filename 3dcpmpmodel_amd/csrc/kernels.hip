@@ -6,7 +6,8 @@
 // wall layers (the geometry of SetFlowField, FluidCuda.cu:204), so no type array
 // is read.  Storage forms: FP64 (double f, rho, mu), FP32 (float), FP16
 // (f stored as f - w_q*rho_ref in _Float16, rho_ref the density that the node
-// pulling the value had in the step that wrote it; rho, mu float).
+// pulling the value had in the step that wrote it; rho, mu float), FP32A (FP64
+// storage as FP64, FP32 arithmetic in the collide).
 #include "internal.h"
 #include "lbm_math.h"
 #include "device_common.h"
@@ -25,6 +26,7 @@ struct Form { using TF = TF_; using TS = TS_; using TC = TC_; };
 using FormF64 = Form<double, double, double>;
 using FormF32 = Form<float, float, float>;
 using FormF16 = Form<_Float16, float, float>;
+using FormF32A = Form<double, double, float>;   // FP32 arithmetic on FP64 storage
 
 template <class TF> constexpr bool kShifted = std::is_same<TF, _Float16>::value;
 
@@ -34,6 +36,7 @@ hipError_t dispatch(int form, Fn &&fn) {
     case CPMP_FORM_FP64: return fn(FormF64{});
     case CPMP_FORM_FP32: return fn(FormF32{});
     case CPMP_FORM_FP16: return fn(FormF16{});
+    case CPMP_FORM_FP32A: return fn(FormF32A{});
     }
     return hipErrorInvalidValue;
 }
@@ -1445,7 +1448,7 @@ int march_chunk(int nx, int ny, int nz, int form) {
         if (v >= 1 && v <= MXC) return v;
     }
     const long long tiles = (long long)((ny + MTY - 1) / MTY) * ((nz + MTZ - 1) / MTZ);
-    if (form != CPMP_FORM_FP64 && (long long)((nx + 1) / 2) * tiles < 4096) return 1;   // small lattice
+    if (!fp64_storage(form) && (long long)((nx + 1) / 2) * tiles < 4096) return 1;   // small lattice
     for (int c = MXC; c >= 2; c /= 2)
         if (nx % c == 0 && (nx / c) * tiles >= 2048) return c;
     for (int c = 2; c <= MXC; c *= 2)
@@ -1465,7 +1468,7 @@ bool node_collide(int nx, int ny, int nz, int form) {
         if (std::strcmp(e, "march") == 0) return false;
     }
     const long long tiles = (long long)((ny + MTY - 1) / MTY) * ((nz + MTZ - 1) / MTZ);
-    return form == CPMP_FORM_FP64 && (long long)((nx + 1) / 2) * tiles < 2048;
+    return fp64_storage(form) && (long long)((nx + 1) / 2) * tiles < 2048;
 }
 
 // previous-density buffer the FP16 form shifts its populations by (null otherwise)

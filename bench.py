@@ -85,8 +85,14 @@ def sessile(cpmp, form, mu_w, steps):
     return out
 
 
+# the forms compared with the FP64 form: FP32 arithmetic on FP64 storage (the EOS mu in
+# FP64 everywhere), FP32 storage and arithmetic (FP64 EOS mu at the interface), FP16
+# storage with FP32 arithmetic
+LOWER = ("fp32a", "fp32", "fp16")
+
+
 def accuracy(cpmp, steps, coalescence_steps=5000):
-    """FP32 and FP16-storage forms against the FP64 form on the same inputs: the
+    """The reduced-precision forms against the FP64 form on the same inputs: the
     sessile droplet at wall chemical potentials calibrated for 30..150 degrees,
     the Laplace jump and spurious current of a periodic droplet, the neck of two
     coalescing droplets.  A wetting case without a contact line in any form is
@@ -101,15 +107,15 @@ def accuracy(cpmp, steps, coalescence_steps=5000):
     log(f"[bench] wall mu for {TARGET_ANGLES} deg: {[round(v, 6) for v in cal.values()]} "
         f"(neutral {mu_b:.6f}), {time.perf_counter() - t0:.1f} s")
     res["wall_mu"] = {str(a): v for a, v in cal.items()}
-    worst = {"fp32": 0.0, "fp16": 0.0}
-    worst_rho = {"fp32": 0.0, "fp16": 0.0}
-    per_angle = {"fp32": {}, "fp16": {}}
+    worst = {f: 0.0 for f in LOWER}
+    worst_rho = {f: 0.0 for f in LOWER}
+    per_angle = {f: {} for f in LOWER}
     invalid = []
     for angle in TARGET_ANGLES:
         mu_w = cal[angle]
         r64 = sessile(cpmp, "fp64", mu_w, steps)
         case = {"target_angle": angle, "mu_w": mu_w, "fp64": r64}
-        for form in ("fp32", "fp16"):
+        for form in LOWER:
             r = sessile(cpmp, form, mu_w, steps)
             case[form] = r
             if not (r64["valid"] and r["valid"]):
@@ -125,19 +131,19 @@ def accuracy(cpmp, steps, coalescence_steps=5000):
             per_angle[form][str(angle)] = err
             worst[form] = max(worst[form], err)
             worst_rho[form] = max(worst_rho[form], rerr)
-        if not all(case[f]["valid"] for f in ("fp64", "fp32", "fp16")):
+        if not all(case[f]["valid"] for f in ("fp64",) + LOWER):
             invalid.append(angle)
         res["cases"].append(case)
         log(f"[bench] wetting {angle} deg (mu_w={mu_w:.6f}): fp64 CaLeft {r64['ca_left']:.2f} CaCap "
-            f"{r64['ca_cap']:.2f} cap3d {r64['cap3d']:.2f} valid {r64['valid']}; fp32 err "
-            f"{per_angle['fp32'][str(angle)]}, fp16 err {per_angle['fp16'][str(angle)]} deg")
+            f"{r64['ca_cap']:.2f} cap3d {r64['cap3d']:.2f} valid {r64['valid']}; err "
+            + ", ".join(f"{f} {per_angle[f][str(angle)]}" for f in LOWER) + " deg")
     res["contact_angle_err_deg_per_angle"] = per_angle
     res["invalid_cases"] = invalid
     res["contact_angle_err_deg"] = None if invalid else worst
     res["coexistence_rel_err"] = None if invalid else worst_rho
     # Laplace jump and spurious current: 64^3 Carnahan-Starling droplet, 1000 steps
     lap = {}
-    for form in ("fp64", "fp32", "fp16"):
+    for form in ("fp64",) + LOWER:
         n = 64
         p = cpmp.make_params(n, n, n, eos="CSE", tr=0.85, nwall=0, Ka=0.01, tau=1.0, form=form, rho0=G_CSE85)
         g = cpmp.Lattice(p)
@@ -148,10 +154,10 @@ def accuracy(cpmp, steps, coalescence_steps=5000):
         lap[form] = {"laplace_dp": dp, "spurious_max_u": s.max_speed}
         g.close()
     res["laplace"] = lap
-    res["laplace_rel_err"] = {f: abs(lap[f]["laplace_dp"] / lap["fp64"]["laplace_dp"] - 1) for f in ("fp32", "fp16")}
+    res["laplace_rel_err"] = {f: abs(lap[f]["laplace_dp"] / lap["fp64"]["laplace_dp"] - 1) for f in LOWER}
     # two-droplet coalescence, 256^3 RK at Tr = 0.6: neck radius in the mid plane and mass drift
     coal = {}
-    for form in ("fp64", "fp32", "fp16"):
+    for form in ("fp64",) + LOWER:
         g = make_lattice(cpmp, 256, form)
         m0 = g.reduce().mass
         g.step(coalescence_steps)
@@ -161,7 +167,7 @@ def accuracy(cpmp, steps, coalescence_steps=5000):
         log(f"[bench] coalescence {form}: neck radius {r:.4f} after {coalescence_steps} steps, "
             f"mass drift {coal[form]['mass_drift']:.2e}")
     res["coalescence"] = {"steps": coalescence_steps, **coal}
-    res["neck_radius_rel_err"] = {f: abs(coal[f]["neck_radius"] / coal["fp64"]["neck_radius"] - 1) for f in ("fp32", "fp16")}
+    res["neck_radius_rel_err"] = {f: abs(coal[f]["neck_radius"] / coal["fp64"]["neck_radius"] - 1) for f in LOWER}
     return res
 
 
@@ -364,7 +370,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,fp64,fp16")
+    ap.add_argument("--forms", default="fp32,fp64,fp16,fp32a")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--sessile-steps", type=int, default=20000)
     ap.add_argument("--coalescence-steps", type=int, default=5000)
@@ -446,9 +452,15 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "step_roofline_frac": round(step_bw / PEAK_HBM_GBS, 4),
+        # per form: the collide's and the step's fraction of the 8 TB/s roofline at the
+        # form's algorithmic population bytes (2 x 19 x storage size per node)
         "forms": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 4),
                       "kernel_ms": {a: round(b, 4) for a, b in v["kernel_ms"].items()},
-                      "bytes_per_node": v["bytes_per_node"], "mass_drift": v["mass_drift"]}
+                      "bytes_per_node": v["bytes_per_node"], "mass_drift": v["mass_drift"],
+                      "collide_roofline_frac": round(v["bytes_per_node"] * nodes / (v["kernel_ms"]["collide"] * 1e-3)
+                                                     / 1e9 / PEAK_HBM_GBS, 4),
+                      "step_roofline_frac": round(v["bytes_per_node"] * nodes / (v["ms_per_step"] * 1e-3)
+                                                  / 1e9 / PEAK_HBM_GBS, 4)}
                   for k, v in res.items()},
         "fp32_over_fp64": round(res["fp32"]["mlups"] / res["fp64"]["mlups"], 3) if "fp64" in res else None,
         "contact_angle_err_deg": acc["contact_angle_err_deg"] if acc else None,

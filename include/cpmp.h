@@ -51,6 +51,8 @@ extern "C" {
 #define CPMP_FORM_FP16 2  /* FP16 populations stored as f_i - w_i*rho_ref, FP32 arithmetic;
                              rho_ref = density the pulling node had in the step that wrote
                              the value (the reference shift w_i*rho0, made per node) */
+#define CPMP_FORM_FP32A 3 /* FP64 populations, rho and mu (the EOS mu in FP64 everywhere),
+                             FP32 collision, force and streaming arithmetic */
 
 /* ---- field ids for get/set ---- */
 #define CPMP_FIELD_TYPE  0  /* 1 fluid, 2 solid (Common.h:37-38); get only */
@@ -259,7 +261,8 @@ int cpmp_adjoint_plan(cpmp_ctx *ctx, const cpmp_adj_desc *d, cpmp_adj_plan *plan
 
 /* ---- measurement ---- */
 /* Average device time per launch (ms) of each kernel over nsteps steps, measured
-   with HIP events on the context's stream.  names: CPMP_K_* order below. */
+   with HIP events on the context's stream (the collide over the first nsteps - 1
+   steps: the last one also writes the MV field).  names: CPMP_K_* order below. */
 #define CPMP_K_WALL    0
 #define CPMP_K_MU      1
 #define CPMP_K_COLLIDE 2

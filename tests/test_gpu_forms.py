@@ -1,5 +1,6 @@
-"""FP32 and FP16-storage forms against the FP64 form, conservation at the full
-256^3 size, checkpoints, Tecplot output, observables."""
+"""The FP32, FP32-arithmetic (FP64 storage) and FP16-storage forms against the FP64
+form, conservation at the full 256^3 size, checkpoints, Tecplot output,
+observables."""
 import os
 
 import numpy as np
@@ -11,7 +12,7 @@ import oracle_lib as orc
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("form,tol", [("fp32", 2e-4), ("fp16", 2e-2)])
+@pytest.mark.parametrize("form,tol", [("fp32", 2e-4), ("fp32a", 2e-4), ("fp16", 2e-2)])
 def test_reduced_precision_forms_track_fp64(cpmp, form, tol):
     case = gc.CASES["ref_prw_walls"]
     _, g64 = gc.build(case, cpmp, "fp64")
@@ -86,7 +87,7 @@ def test_interface_chemical_potential_in_fp64(cpmp, monkeypatch, form, geom):
         assert ulps[band].max() <= 1.5
 
 
-@pytest.mark.parametrize("form", ["fp64", "fp32", "fp16"])
+@pytest.mark.parametrize("form", ["fp64", "fp32", "fp16", "fp32a"])
 def test_mass_conservation_256cubed(cpmp, form):
     """Full-size property: a periodic 256^3 RK coalescence run conserves mass
     (FP64 sums on the device) and stays finite."""
@@ -100,8 +101,39 @@ def test_mass_conservation_256cubed(cpmp, form):
     drift = abs(s.mass - m0) / m0
     print(f"[256^3 {form}] mass drift after 20 steps {drift:.3e}, max|u| {s.max_speed:.3e}", flush=True)
     assert s.bad_count == 0
-    assert drift < {"fp64": 1e-12, "fp32": 1e-5, "fp16": 1e-3}[form]
+    assert drift < {"fp64": 1e-12, "fp32": 1e-5, "fp16": 1e-3, "fp32a": 1e-5}[form]
     g.close()
+
+
+@pytest.mark.parametrize("geom", ["walls", "periodic"])
+def test_fp32_arithmetic_form_keeps_fp64_storage_and_mu(cpmp, monkeypatch, geom):
+    """FP32A: populations, rho and mu stored in FP64 and mu evaluated in FP64 (the
+    chemical-potential code of the FP64 form: bit-identical mu for the same density),
+    the collision, force and streaming arithmetic in FP32 (one step differs from the
+    FP64 form's by FP32 rounding, not by more).  Periodic: the fused density sum with
+    chunks of 8 planes."""
+    if geom == "walls":
+        case = gc.CASES["ref_prw_walls"]
+    else:
+        case = dict(gc.CASES["cse_periodic"], dims=(64, 32, 64))
+        monkeypatch.setenv("CPMP_XC", "8")
+        monkeypatch.setenv("CPMP_COLLIDE", "march")
+    _, g64 = gc.build(case, cpmp, "fp64")
+    _, ga = gc.build(case, cpmp, "fp32a")
+    assert ga.algorithmic_bytes_per_node() == 2 * 19 * 8    # FP64 populations
+    r0 = ga.get("rho")
+    assert np.array_equal(r0, g64.get("rho"))
+    g64.step(1)
+    ga.step(1)
+    # mu of the same initial density, FP64 in both
+    assert np.array_equal(ga.get("mu"), g64.get("mu"))
+    fluid = g64.get("type") == 1
+    r64, ra = g64.get("rho")[fluid], ga.get("rho")[fluid]
+    d = np.max(np.abs(ra - r64) / r64)
+    print(f"[fp32a] {geom}: one step, max rel |rho - rho_fp64| = {d:.3e}", flush=True)
+    assert 0 < d < 5e-6
+    g64.close()
+    ga.close()
 
 
 def test_checkpoint_roundtrip_and_corruption(cpmp, tmp_path):

@@ -112,17 +112,20 @@ def test_fp64_march_long_chunks_match_oracle(cpmp, name, xc):
     g.close()
 
 
-@pytest.mark.parametrize("form,tol", [("fp32", 2e-4), ("fp16", 2e-2)])
+@pytest.mark.parametrize("form,tol", [("fp32", 2e-4), ("fp16", 2e-2), ("fp32a", 2e-4)])
 @pytest.mark.parametrize("name", list(GEOMS))
 def test_reduced_forms_long_chunks_match_oracle(cpmp, name, form, tol):
     """fused density with chunks of 32 planes (periodic FP32: the completion folded
-    into the next step's chemical-potential kernel) against the FP64 oracle"""
+    into the next step's chemical-potential kernel) against the FP64 oracle; the
+    FP32-arithmetic form (FP64 storage) forced onto the march collide, which these
+    lattices would otherwise run one thread per node as the FP64 form does"""
     case = GEOMS[name]
     print(f"[production] {form} {name} xc=32 steps={case['steps']}", flush=True)
     rho0, rho_o, _, fluid = oracle_run(name)
-    g = device(name, cpmp, form, CPMP_XC=32)
-    short = device(name, cpmp, form, CPMP_XC=2)
-    unfused = device(name, cpmp, form, CPMP_XC=32, CPMP_FUSE=0)
+    env = dict(CPMP_COLLIDE="march") if form == "fp32a" else {}
+    g = device(name, cpmp, form, CPMP_XC=32, **env)
+    short = device(name, cpmp, form, CPMP_XC=2, **env)
+    unfused = device(name, cpmp, form, CPMP_XC=32, CPMP_FUSE=0, **env)
     assert g.memory() > unfused.memory()   # the fused path
     unfused.close()
     # rho is the sum of the pulled populations after one step and a run of five

@@ -60,6 +60,12 @@ size_t band_slots(const cpmp_ctx *c) {
     const size_t blocks = (size_t)((c->p.nz + kTileZ - 1) / kTileZ) * ((c->p.ny + kTileY - 1) / kTileY) * ((c->p.nx + xc - 1) / xc);
     return blocks * kTileY * kTileZ * xc;
 }
+// overflow descriptors: a run of n entries posts ceil((n - kBandWave) / kBandDesc) <=
+// n / kBandDesc + 1, so slots / kBandDesc + (runs) bound their number
+size_t band_desc_slots(const cpmp_ctx *c) {
+    const size_t runs = band_slots(c) / (64 * (size_t)c->dp.xc);
+    return band_slots(c) / kBandDesc + runs + 1;
+}
 
 void fill_devparams(cpmp_ctx *c) {
     const cpmp_params &p = c->p;
@@ -301,8 +307,15 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
         if (hipMalloc(&c->rho_next, c->N * c->scal_bytes) != hipSuccess) return fail(CPMP_ENOMEM, "previous rho");
         c->fuse_bytes = (size_t)c->N * c->scal_bytes;
     }
-    if (!fp64_storage(c->form) && hipMalloc((void **)&c->band, band_slots(c) * sizeof(uint4)) != hipSuccess)
-        return fail(CPMP_ENOMEM, "interface runs");
+    if (!fp64_storage(c->form)) {
+        if (hipMalloc((void **)&c->band, band_slots(c) * sizeof(uint4)) != hipSuccess)
+            return fail(CPMP_ENOMEM, "interface runs");
+        // descriptors, then the two counters
+        if (hipMalloc((void **)&c->band_ovf, (band_desc_slots(c) + 1) * sizeof(uint2)) != hipSuccess)
+            return fail(CPMP_ENOMEM, "interface overflow");
+        c->band_cnt = (unsigned *)(c->band_ovf + band_desc_slots(c));
+        c->band_cap = (unsigned)band_desc_slots(c);
+    }
     if (hipMalloc((void **)&c->mv, 3 * c->N * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "MV");
     if (hipMalloc((void **)&c->mu_w, 2 * c->NXY * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "wall mu");
     if (hipMalloc((void **)&c->red, reduce_scratch_doubles() * sizeof(double)) != hipSuccess) return fail(CPMP_ENOMEM, "reduce");
@@ -311,6 +324,7 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
         hipMemsetAsync(c->rho, 0, c->N * c->scal_bytes, c->stream) != hipSuccess ||
         hipMemsetAsync(c->mu, 0, c->N * c->scal_bytes, c->stream) != hipSuccess ||
         (c->rho_next && hipMemsetAsync(c->rho_next, 0, c->N * c->scal_bytes, c->stream) != hipSuccess) ||
+        (c->band_cnt && hipMemsetAsync(c->band_cnt, 0, 2 * sizeof(unsigned), c->stream) != hipSuccess) ||
         hipMemsetAsync(c->mv, 0, 3 * c->N * sizeof(double), c->stream) != hipSuccess ||
         hipMemsetAsync(c->mu_w, 0, 2 * c->NXY * sizeof(double), c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
@@ -322,8 +336,8 @@ int cpmp_create(const cpmp_params *p, cpmp_ctx **out) {
 int cpmp_destroy(cpmp_ctx *c) {
     if (!c) return CPMP_OK;
     if (c->stream) { hipSetDevice(c->p.device); hipStreamSynchronize(c->stream); }
-    for (void *ptr : {c->f[0], c->f[1], c->rho, c->rho_next, c->rbuf, c->cbuf, c->mu, (void *)c->band, (void *)c->mv,
-                      (void *)c->mu_w, (void *)c->red, (void *)c->staging})
+    for (void *ptr : {c->f[0], c->f[1], c->rho, c->rho_next, c->rbuf, c->cbuf, c->mu, (void *)c->band,
+                      (void *)c->band_ovf, (void *)c->mv, (void *)c->mu_w, (void *)c->red, (void *)c->staging})
         if (ptr) hipFree(ptr);
     if (c->timers) {
         for (auto &e : c->timers->ev) if (e) hipEventDestroy(e);
@@ -344,7 +358,7 @@ int cpmp_memory(cpmp_ctx *c, int64_t *bytes) {
     if (!valid_ctx(c) || !bytes) return set_error(CPMP_EINVAL, "bad context or output");
     *bytes = (int64_t)(2 * (size_t)c->N * 19 * c->pop_bytes + 2 * c->N * c->scal_bytes + 4 * c->N * sizeof(double)
                        + 2 * c->NXY * sizeof(double) + reduce_scratch_doubles() * sizeof(double) + c->fuse_bytes
-                       + (c->band ? band_slots(c) * sizeof(uint4) : 0));
+                       + (c->band ? band_slots(c) * sizeof(uint4) + (band_desc_slots(c) + 1) * sizeof(uint2) : 0));
     return CPMP_OK;
 }
 

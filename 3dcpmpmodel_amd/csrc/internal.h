@@ -62,6 +62,14 @@ struct cpmp_ctx {
     // FP32 storage forms: per wave of the chemical-potential march, a run of 64 * xc
     // slots (node, density, Laplacian term) for the interface nodes it finishes in FP64
     uint4 *band = nullptr;
+    // the entries of a run beyond the first kBandWave: descriptors (first slot, count of
+    // at most kBandDesc) that k_band_overflow finishes after the march, and two counters
+    // of posted descriptors used by alternate launches (band_par), each cleared by the
+    // overflow kernel of the launch before its use
+    uint2 *band_ovf = nullptr;
+    unsigned *band_cnt = nullptr;
+    unsigned band_cap = 0;      // descriptor slots
+    int band_par = 0;
     bool collide_v1 = false;    // one-thread-per-node collide (node_collide)
     size_t fuse_bytes = 0;
     double *mv = nullptr;       // 3*N, MV of the last step
@@ -85,6 +93,8 @@ bool density_fused(const cpmp_ctx *c);
 bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *chunk_elems);
 // forms whose populations, rho and mu are stored in FP64 (FP64, FP32A)
 inline bool fp64_storage(int form) { return form == CPMP_FORM_FP64 || form == CPMP_FORM_FP32A; }
+constexpr unsigned kBandWave = 128;   // interface nodes a wave of the mu march finishes itself
+constexpr unsigned kBandDesc = 256;   // interface nodes per overflow descriptor (one workgroup)
 // x planes per march workgroup for a lattice and storage form (CPMP_XC overrides)
 int march_chunk(int nx, int ny, int nz, int form);
 // the one-thread-per-node collide instead of the x march (CPMP_COLLIDE=v1 / march

@@ -275,12 +275,50 @@ __device__ __forceinline__ void band_append(bool in, unsigned idx, float d, floa
         make_uint4(idx, __float_as_uint(d), __float_as_uint(kal), 0u);
     count += (unsigned)__popcll(m);
 }
-__device__ __forceinline__ void band_finish(const DevParams &P, const uint4 *run, unsigned count, float *mu) {
+// A wave finishes the first kBandWave entries of its run; the rest (the interface
+// crosses some waves' 64 columns for most of their planes: up to ~1800 entries at 256^3
+// against a mean of ~70 per wave) goes to k_band_overflow in descriptors of kBandDesc
+// entries, so that no wave holds the end of the launch for tens of rounds of the FP64
+// EOS.  run_index: the run's first slot in the band array; ovf_cap: the descriptor
+// slots (never exceeded while each launch's counter starts at zero; past it a wave
+// marks its reserved slots empty and finishes all its entries itself).
+__device__ __forceinline__ void band_finish(const DevParams &P, const uint4 *run, unsigned count, float *mu,
+                                            unsigned run_index, uint2 *ovf, unsigned *ovf_cnt, unsigned ovf_cap) {
     if (count == 0) return;   // uniform over the wave
     __threadfence_block();    // entries written by other lanes are visible
+    if (count > kBandWave) {  // uniform
+        const unsigned extra = count - kBandWave, nd = (extra + kBandDesc - 1) / kBandDesc;
+        unsigned d0 = 0;
+        if (__lane_id() == 0) d0 = atomicAdd(ovf_cnt, nd);
+        d0 = (unsigned)__shfl((int)d0, 0);
+        const bool fits = d0 <= ovf_cap && nd <= ovf_cap - d0;
+        for (unsigned j = __lane_id(); j < nd && d0 + j < ovf_cap; j += 64)
+            ovf[d0 + j] = fits ? make_uint2(run_index + kBandWave + kBandDesc * j, min(kBandDesc, extra - kBandDesc * j))
+                               : make_uint2(0u, 0u);
+        if (fits) count = kBandWave;
+    }
     for (unsigned i = __lane_id(); i < count; i += 64) {
         const uint4 e = run[i];
         mu[e.x] = mu_band(P, __uint_as_float(e.y), __uint_as_float(e.z));
+    }
+}
+
+// The interface entries beyond the first kBandWave of the runs of one mu launch: one
+// workgroup per descriptor of at most kBandDesc entries.  Also clears the descriptor
+// counter that the next launch posts to (cnt_next), so that no memset is needed.
+__global__ void __launch_bounds__(kBandDesc) k_band_overflow(const DevParams P, const uint4 *__restrict__ band,
+                                                             const uint2 *__restrict__ ovf,
+                                                             const unsigned *__restrict__ cnt,
+                                                             unsigned *__restrict__ cnt_next, float *__restrict__ mu,
+                                                             unsigned cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cnt_next = 0u;
+    const unsigned n = min(*cnt, cap);
+    for (unsigned d = blockIdx.x; d < n; d += gridDim.x) {
+        const uint2 ds = ovf[d];
+        if (threadIdx.x < ds.y) {
+            const uint4 e = band[ds.x + threadIdx.x];
+            mu[e.x] = mu_band(P, __uint_as_float(e.y), __uint_as_float(e.z));
+        }
     }
 }
 
@@ -517,7 +555,9 @@ template <class TS, bool FIX, int H>
 __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__restrict__ rho, TS *__restrict__ mu,
                                                   int ntz, int nty, TS *__restrict__ rho_solid, int xc,
                                                   const TS *__restrict__ rbuf, const TS *__restrict__ cbuf,
-                                                  TS *__restrict__ rho_out, int nch, uint4 *__restrict__ band) {
+                                                  TS *__restrict__ rho_out, int nch, uint4 *__restrict__ band,
+                                                  uint2 *__restrict__ band_ovf, unsigned *__restrict__ band_cnt,
+                                                  unsigned band_cap) {
     // ring of 2H+2 planes with an in-plane halo of H (2 for the 5-point Laplacian, 3 for
     // the 7-point one): the stencil reads 2H+1 of them, and the extra slot lets the
     // plane entering the ring be stored while slower waves still read the plane that
@@ -525,7 +565,8 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
     constexpr int R = 2 * H + 2, LZ = MTZ + 2 * H, LP = (MTY + 2 * H) * LZ;
     __shared__ TS s_rho[R][LP];
     // FP32: this wave's run of interface nodes and their count
-    uint4 *band_run = band ? band + ((size_t)blockIdx.x * (MT / 64) + threadIdx.x / 64) * 64 * xc : nullptr;
+    const unsigned run_index = (unsigned)(((size_t)blockIdx.x * (MT / 64) + threadIdx.x / 64) * 64 * xc);
+    uint4 *band_run = band ? band + run_index : nullptr;
     unsigned band_count = 0;
     int t = blockIdx.x;
     if ((gridDim.x & 7) == 0) t = (t & 7) * (gridDim.x >> 3) + (t >> 3);
@@ -688,7 +729,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
     }
     if (x < xb) plane_step(x, hA, fA);
     if constexpr (sizeof(TS) == 4)
-        if (band) band_finish(P, band_run, band_count, (float *)mu);
+        if (band) band_finish(P, band_run, band_count, (float *)mu, run_index, band_ovf, band_cnt, band_cap);
 }
 
 // With RS the kernel also sums the next step's density: every post-collision
@@ -1476,6 +1517,20 @@ template <class TS> static const TS *ref_buffer(const cpmp_ctx *c) {
     return c->form == CPMP_FORM_FP16 ? (const TS *)c->rho_next : nullptr;
 }
 
+// after a mu march with interface runs: the entries its waves posted as overflow, then
+// the other descriptor counter for the next launch
+static hipError_t launch_band_overflow(cpmp_ctx *c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int par = c->band_par;
+    // a grid of 2048 workgroups (8 per CU) strides over the descriptors: ~2100 at 256^3
+    hipLaunchKernelGGL(k_band_overflow, dim3(2048), dim3(kBandDesc), 0, c->stream, c->dp, (const uint4 *)c->band,
+                       (const uint2 *)c->band_ovf, (const unsigned *)(c->band_cnt + par), c->band_cnt + (par ^ 1),
+                       (float *)c->mu, c->band_cap);
+    c->band_par = par ^ 1;
+    return hipGetLastError();
+}
+
 hipError_t launch_mu(cpmp_ctx *c, void *rho_copy) {
     return dispatch(c->form, [&](auto fm) {
         using TS = typename decltype(fm)::TS;
@@ -1494,10 +1549,12 @@ hipError_t launch_mu(cpmp_ctx *c, void *rho_copy) {
             auto go = [&](auto h_t) {
                 hipLaunchKernelGGL((k_mu_march<TS, false, decltype(h_t)::value>), dim3((unsigned)(ntz * nty * nch)),
                                    dim3(MT), 0, c->stream, c->dp, (const TS *)c->rho, (TS *)c->mu, ntz, nty, solid, xc,
-                                   (const TS *)nullptr, (const TS *)nullptr, (TS *)rho_copy, nch, c->band);
+                                   (const TS *)nullptr, (const TS *)nullptr, (TS *)rho_copy, nch, c->band, c->band_ovf,
+                                   c->band_cnt ? c->band_cnt + c->band_par : nullptr, c->band_cap);
             };
             if (c->dp.lap_points == 7) go(std::integral_constant<int, 3>{});
             else go(std::integral_constant<int, 2>{});
+            if (c->band) return launch_band_overflow(c);
         }
         return hipGetLastError();
     });
@@ -1635,10 +1692,12 @@ static hipError_t launch_mu_fix(cpmp_ctx *c) {
     auto go = [&](auto h_t) {
         hipLaunchKernelGGL((k_mu_march<float, true, decltype(h_t)::value>), dim3((unsigned)(ntz * nty * nch)), dim3(MT), 0,
                            c->stream, c->dp, (const float *)c->rho_next, (float *)c->mu, ntz, nty, (float *)nullptr, xc,
-                           (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch, c->band);
+                           (const float *)c->rbuf, (const float *)c->cbuf, (float *)c->rho, nch, c->band, c->band_ovf,
+                           c->band_cnt ? c->band_cnt + c->band_par : nullptr, c->band_cap);
     };
     if (c->dp.lap_points == 7) go(std::integral_constant<int, 3>{});
     else go(std::integral_constant<int, 2>{});
+    if (c->band) return launch_band_overflow(c);
     return hipGetLastError();
 }
 

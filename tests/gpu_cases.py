@@ -1,8 +1,16 @@
 """Shared case definitions for the GPU parity tests: each builds the same
 lattice in the oracle and in the HIP library from one description."""
+import os
+
 import numpy as np
 
 import oracle_lib as orc
+
+
+def path_forced():
+    """the environment selects the collide or density path itself (CPMP_FUSE=0,
+    CPMP_COLLIDE=v1): assertions about the default path selection do not apply"""
+    return os.environ.get("CPMP_FUSE") == "0" or os.environ.get("CPMP_COLLIDE") == "v1"
 
 RHO = {("PRW", 0.70): (0.0556211, 8.08045), ("PRW", 0.75): (0.108623, 7.67892),
        ("CSE", 0.85): (0.0315818, 0.279262), ("RKE", 0.60): (0.0319804, 8.25467),

@@ -159,6 +159,8 @@ def test_small_tiled_lattices_select_the_separate_pass(cpmp, name):
 def test_fused_path_taken_on_tiled_lattices(cpmp, name):
     """the fused contexts hold the extra density buffers exactly when the tiles
     cover the lattice"""
+    if gc.path_forced():
+        pytest.skip("the environment overrides the path selection")
     a = make(cpmp, name, "fp32", fuse=True)
     b = make(cpmp, name, "fp32", fuse=False)
     extra = a.memory() - b.memory()
@@ -174,7 +176,8 @@ def test_tiled_layout_fields_checkpoints_and_repeatability(cpmp, tmp_path):
     name = "tiled_prw_64x16x64"
     a = make(cpmp, name, "fp32", fuse=True)
     b = make(cpmp, name, "fp32", fuse=False)
-    assert a.memory() > b.memory()
+    if not gc.path_forced():
+        assert a.memory() > b.memory()
     assert np.array_equal(a.get("f"), b.get("f"))
     a.step(7)
     path = str(tmp_path / "tiled.ckp")

@@ -126,7 +126,8 @@ def test_reduced_forms_long_chunks_match_oracle(cpmp, name, form, tol):
     g = device(name, cpmp, form, CPMP_XC=32, **env)
     short = device(name, cpmp, form, CPMP_XC=2, **env)
     unfused = device(name, cpmp, form, CPMP_XC=32, CPMP_FUSE=0, **env)
-    assert g.memory() > unfused.memory()   # the fused path
+    if not gc.path_forced():
+        assert g.memory() > unfused.memory()   # the fused path
     unfused.close()
     # rho is the sum of the pulled populations after one step and a run of five
     for n in (1, 5):

@@ -582,7 +582,6 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
     const bool active = inr && !is_solid(P, z);
     constexpr int HE = (LP + MT - 1) / MT;
     int hoff[HE];
-    bool hown[HE];
     // FIX: in-plane part of the ring index of the (up to three) neighbouring tiles
     // holding each halo node (y-, z-, diagonal neighbour), -1 where absent
     int ro[FIX ? HE : 1][3];
@@ -597,7 +596,6 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         const int hy = e < LP ? wrapc(y0 - H + ly, P.ny) : 0;
         const int hz = e < LP ? wrapc(z0 - H + lz, P.nz) : 0;
         hoff[h] = hy * P.nz + hz;
-        hown[h] = e < LP && ly >= H && ly < H + MTY && lz >= H && lz < H + MTZ;
         if constexpr (FIX) {
             const int gyi = hy / MTY, gzi = hz / MTZ, gy = hy % MTY, gz = hz % MTZ;
             const bool ey0 = gy == 0, ey1 = gy == MTY - 1, ez0 = gz == 0, ez1 = gz == MTZ - 1;
@@ -651,8 +649,8 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         st_k = st_xw % xc;
         st_ch = st_xw / xc;
     }
-    auto store_plane = [&](int slot, int xg, const TS *hr, TS (*fv)[3]) {
-        const int xw = st_xw, k = st_k;
+    auto store_plane = [&](int slot, const TS *hr, TS (*fv)[3]) {
+        const int k = st_k;
         const bool chunk_end = k == 0 || k == xc - 1;   // partial-plane terms too (uniform)
 #pragma unroll
         for (int h = 0; h < HE; ++h)
@@ -678,7 +676,6 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
                         a += fv[h][2];
                     }
                     v = v + a;
-                    if (hown[h] && xg >= xa && xg < xb) rho_out[(long long)xw * P.ny * P.nz + hoff[h]] = v;
                 }
                 s_rho[slot][threadIdx.x + h * MT] = v;
             }
@@ -691,7 +688,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
             }
         }
     };
-    for (int k = -H; k < H; ++k) { fetch_plane(xa + k, hA, fA); store_plane(k + H, xa + k, hA, fA); }
+    for (int k = -H; k < H; ++k) { fetch_plane(xa + k, hA, fA); store_plane(k + H, hA, fA); }
     fetch_plane(xa + H, hA, fA);
     fetch_plane(xa + H + 1, hB, fB);
     const int base = (ty + H) * LZ + tz + H;
@@ -702,7 +699,7 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
         int slot[R];
 #pragma unroll
         for (int dd = 0; dd < R; ++dd) slot[dd] = rs0 + dd >= R ? rs0 + dd - R : rs0 + dd;
-        store_plane(slot[2 * H], x + H, hr, fv);             // plane x+H
+        store_plane(slot[2 * H], hr, fv);                    // plane x+H
         __syncthreads();
         fetch_plane(x + 2 + H, hr, fv);                      // plane x+H+2
         const long long I = ((long long)x * P.ny + ym) * P.nz + zm;
@@ -718,8 +715,9 @@ __global__ void __launch_bounds__(MT) k_mu_march(const DevParams P, const TS *__
                 const float dcen = s_rho[slot[H]][base];
                 band_append(active && in_band(P, dcen), (unsigned)I, dcen, kal, band_run, band_count);
             }
-        // without FIX, rho_out (if given) receives a copy of rho (the adjoint's tape)
-        if (!FIX && rho_out && inr) rho_out[I] = s_rho[slot[H]][base];
+        // rho_out (if given) receives the density of the node: with FIX the completed
+        // one, without a copy of rho (the adjoint's tape)
+        if (rho_out && inr) rho_out[I] = s_rho[slot[H]][base];
         rs0 = rs0 + 1 == R ? 0 : rs0 + 1;
     };
     int x = xa;

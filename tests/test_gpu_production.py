@@ -30,6 +30,11 @@ GEOMS = {
     # PR droplet on 3 + 3 wall layers (the reference geometry)
     "walled": dict(dims=(64, 16, 64), eos="PRW", tr=0.70, nwall=3, init="reference", radius=13, width=5,
                    mu_w=-0.02, steps=200),
+    # the periodic case with the 7-point Laplacian (PTCALCSEVENPOINT, Common.h:243-246):
+    # halo of 3 in the chemical-potential march and in the folded completion
+    "periodic7": dict(dims=(64, 16, 64), eos="RKE", tr=0.60, nwall=0, init="spheres",
+                      spheres=[(20, 8, 31.5, 10), (43, 8, 33, 10)], width=5, noise=1e-4, seed=3, steps=200,
+                      lap_points=7),
 }
 
 
@@ -151,11 +156,14 @@ def test_reduced_forms_long_chunks_match_oracle(cpmp, name, form, tol):
     short.close()
 
 
-def test_folded_completion_long_chunks_is_bitwise_the_same(cpmp):
-    """periodic FP32 with chunks of 32: the completion folded into the next mu
-    kernel leaves the same bits as the completion pass after every step"""
-    a = device("periodic", cpmp, "fp32", CPMP_XC=32)
-    b = device("periodic", cpmp, "fp32", CPMP_XC=32)
+@pytest.mark.parametrize("name", ["periodic", "periodic7"])
+def test_folded_completion_long_chunks_is_bitwise_the_same(cpmp, name):
+    """periodic FP32 with chunks of 32 (5- and 7-point Laplacian): the completion
+    folded into the next mu kernel leaves the same bits as the completion pass
+    after every step"""
+    print(f"[production] folded completion {name}", flush=True)
+    a = device(name, cpmp, "fp32", CPMP_XC=32)
+    b = device(name, cpmp, "fp32", CPMP_XC=32)
     for n in (1, 4, 9):
         with env(CPMP_DEFER=0):
             b.step(n)

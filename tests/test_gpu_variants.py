@@ -1,4 +1,4 @@
-"""The Guo force term and ChemBoundaryComplex on the device.
+"""The Guo force term, ChemBoundaryComplex and the FP32-arithmetic form on the device.
 
 FP64 parity of both (<= 1e-10 on rho) runs with every case of gpu_cases.CASES in
 test_gpu_parity.py; here the FP32 and FP16 storage forms (fused density with
@@ -93,9 +93,11 @@ def test_complex_wall_stage_fields(cpmp, lo, hi):
 
 
 def test_adjoint_refuses_variants(cpmp):
+    """the adjoint is derived for the exact-difference force term, the layered and
+    simple wall schemes, and the FP64 and FP32 forms"""
     dims = (16, 12, 20)
     target = np.zeros(dims[0] * dims[1] * dims[2])
-    for kw in (dict(forcing=1), dict(wall_scheme=2)):
+    for kw in (dict(forcing=1), dict(wall_scheme=2), dict(form="fp32a")):
         g = cpmp.Lattice(cpmp.make_params(*dims, eos="PRW", tr=0.75, **kw))
         g.set_wall_mu(0.0)
         g.init_reference(5, 4, 0.108623, 7.67892)

@@ -1473,27 +1473,32 @@ bool fuse_geometry(int nx, int ny, int nz, int xc, size_t *ring_elems, size_t *c
 }
 
 // Planes per workgroup of the march kernels: the longest chunk (32, 16, ..., 2)
-// that divides nx and still gives 2048 workgroups (8 per CU); small lattices get
-// short chunks so that they fill the GPU.  CPMP_XC overrides.
-// The longest chunk of 32..2 planes that gives 2048 workgroups.  FP32 storage forms on
-// lattices with fewer than 4096 workgroups of 2 planes take single planes, which leaves
-// the fused density path (it needs chunks of 2 or more) for the separate density pass:
-// measured 6-18 % faster per step at 64^3, 96x96x64 and 128x128x96, slower from 128^3 on
-// (FP64 lattices below 2048 workgroups of 2 planes take the per-node collide instead,
-// node_collide).
+// that divides nx and still gives 1024 workgroups.  A longer chunk loads a
+// smaller share of halo planes and chunk-end partial planes; below about 1024
+// workgroups the launches no longer spread evenly over the CUs.  Measured per
+// step against the longest chunk giving 2048 workgroups (FP32 / FP16 / FP64):
+// 128^3 -12 / -8 / -8 %, 192^3 -3 / -8 / -4 %, 32x256x256 -12 / -7 / -12 %,
+// 256x256x32 -15 / -11 / -13 %, 128x128x96 with walls (FP64) -13 %; 256^3 and
+// 512x512x256 keep 32.  FP32 storage forms on lattices with fewer than 2048
+// workgroups of 2 planes take single planes, which leaves the fused density path
+// (it needs chunks of 2 or more) for the separate density pass: 6 % faster per
+// step than chunks of 2 at 64^3, 16 % at 96x96x64; 128x128x96 (3072 such
+// workgroups) takes chunks of 4, 8 % faster than single planes in FP32 and FP16
+// (FP64 lattices below 2048 workgroups of 2 planes take the per-node collide
+// instead, node_collide).  CPMP_XC overrides.
 int march_chunk(int nx, int ny, int nz, int form) {
     if (const char *e = std::getenv("CPMP_XC")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= MXC) return v;
     }
     const long long tiles = (long long)((ny + MTY - 1) / MTY) * ((nz + MTZ - 1) / MTZ);
-    if (!fp64_storage(form) && (long long)((nx + 1) / 2) * tiles < 4096) return 1;   // small lattice
+    if (!fp64_storage(form) && (long long)((nx + 1) / 2) * tiles < 2048) return 1;   // small lattice
     for (int c = MXC; c >= 2; c /= 2)
-        if (nx % c == 0 && (nx / c) * tiles >= 2048) return c;
+        if (nx % c == 0 && (nx / c) * tiles >= 1024) return c;
     for (int c = 2; c <= MXC; c *= 2)
         if (nx % c == 0) return c;
     for (int c = MXC; c >= 2; c /= 2)
-        if ((nx + c - 1) / c * tiles >= 2048) return c;
+        if ((nx + c - 1) / c * tiles >= 1024) return c;
     return 2;
 }
 bool density_fused(const cpmp_ctx *c) { return c->rbuf && !c->collide_v1; }

@@ -323,6 +323,44 @@ def large_run(cpmp, fwd_steps=10000, adj_steps=2000, every=100):
     return out
 
 
+XGMI_LINK_GBS = 153.0     # one MI355X xGMI link, one direction
+
+
+def shard_step(cpmp, n=256, gpus=8, steps=200, warmup=20):
+    """One GPU's share of the 256^3 FP32 step split into x slabs over `gpus` GPUs:
+    the slab x in [64, 64 + n/gpus) of the two-droplet field (it cuts the first
+    droplet through its centre), run as a periodic lattice of that shape, which
+    stands in for the halo planes the neighbours would send.  Reports the slab's
+    step time and kernels beside the bytes one step would exchange per face (the 5
+    populations with c_x = +-1 of the boundary plane and 5 planes of rho: radius 2
+    for mu and 3 for the force) and their time on one xGMI link."""
+    import numpy as np
+    nxs = n // gpus
+    x0 = 64
+    x = np.arange(x0, x0 + nxs, dtype=np.float64)
+    y = np.arange(n, dtype=np.float64)
+    X, Y, Z = np.meshgrid(x, y, y, indexing="ij")
+    phi = np.zeros((nxs, n, n))
+    for cx in (80 * n / 256, 176 * n / 256):
+        r = np.sqrt((X - cx) ** 2 + (Y - n / 2) ** 2 + (Z - n / 2) ** 2)
+        phi = np.maximum(phi, 0.5 * (1 - np.tanh(2 * (r - 48 * n / 256) / 5.0)))
+    rho = G_RKE + (L_RKE - G_RKE) * phi
+    g = cpmp.Lattice(cpmp.make_params(nxs, n, n, eos="RKE", tr=0.6, nwall=0, form="fp32"))
+    g.init_from_rho(rho.ravel())
+    g.step(warmup)
+    t0 = time.perf_counter()
+    g.step(steps)
+    dt = time.perf_counter() - t0
+    ktimes, _ = g.kernel_times(10)
+    g.close()
+    nodes = nxs * n * n
+    face_bytes = (5 * 4 + 5 * 4) * n * n
+    return {"config": f"{nxs}x{n}x{n} FP32 x slab (planes {x0}..{x0 + nxs - 1} of the {n}^3 two-droplet field), "
+                      f"periodic in place of the halo exchange, one GPU of {gpus}",
+            "ms_per_step": dt / steps * 1e3, "mlups": nodes * steps / dt / 1e6, "kernel_ms": ktimes,
+            "exchange_bytes_per_face": face_bytes, "exchange_us_one_link": face_bytes / (XGMI_LINK_GBS * 1e9) * 1e6}
+
+
 def cpu_baseline(n, steps):
     """The FP64 OpenMP oracle (CPU restatement of the reference step) on the host
     cores, on a bounded sample: the same 256^3 workload for `steps` steps."""
@@ -377,6 +415,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true", help="run the adjoint cost leg also with --no-accuracy")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 512x512x256 FP32 wetting run with adjoint (about 75 s, ~270 GB)")
+    ap.add_argument("--no-shard", action="store_true",
+                    help="skip the 32x256x256 x slab (one GPU's share of the 256^3 lattice over 8 GPUs)")
     ap.add_argument("--large-fwd", type=int, default=10000, help="forward steps of the large run")
     ap.add_argument("--large-adj", type=int, default=2000, help="steps the large run's adjoint covers")
     ap.add_argument("--large-every", type=int, default=100, help="checkpoint interval of the large run's adjoint")
@@ -426,6 +466,12 @@ def main():
         adj = adjoint_cost(cpmp)
         log(f"[bench] adjoint/forward = {adj['adjoint_over_forward']:.2f} (checkpoints + recompute), "
             f"{adj['adjoint_taped_over_forward']:.2f} (tape in device memory)")
+    shard = None
+    if n == 256 and "fp32" in res and not args.no_shard:
+        shard = shard_step(cpmp, n)
+        log(f"[bench] shard {shard['config']}: {shard['mlups']:.1f} MLUPS, {shard['ms_per_step']:.4f} ms/step, "
+            f"kernels {shard['kernel_ms']}; exchange {shard['exchange_bytes_per_face'] / 1e6:.2f} MB per face, "
+            f"{shard['exchange_us_one_link']:.1f} us on one link")
     large = None
     if not args.no_large:
         large = large_run(cpmp, args.large_fwd, args.large_adj, args.large_every)
@@ -471,6 +517,7 @@ def main():
         "adjoint_taped_over_forward": round(adj["adjoint_taped_over_forward"], 3) if adj else None,
         "accuracy": acc,
         "adjoint": adj,
+        "shard": shard,
         "large": large,
         "cpu_baseline": cpu,
     }

@@ -1,15 +1,16 @@
 """The march configuration that large lattices run, against the oracle.
 
 Lattices of 256^3 and 512x512x256 select x-march workgroups of 32 planes in every
-storage form.  Here lattices with nx >= 64 run with chunks of 8, 16 and 32 planes
-(CPMP_XC), so the 7-slot LDS ring rotates over long chunks, the chemical-potential
-march alternates its two register sets over many planes and the chunk-end partial
-planes of the summed density are exercised, in the FP64 form (<= 1e-10 relative on
-rho after 200 steps), the FP32 form with the fused density and the completion
-folded into the next chemical-potential kernel, and the FP16 form; then every EOS
-in the FP32 form (including the FP32 bulk chemical potential), and the large
-van der Waals striped-wall wetting configuration at a size the oracle covers,
-with its adjoint."""
+storage form, 128x128x96 and similar lattices 4.  Here lattices with nx >= 64 run
+with chunks of 4, 8, 16 and 32 planes (CPMP_XC), so the 8-slot LDS ring rotates
+over long chunks, the chemical-potential march alternates its two register sets
+over many planes and the chunk-end partial planes of the summed density are
+exercised, in the FP64 form (<= 1e-10 relative on rho after 200 steps), the FP32
+form with the fused density and the completion folded into the next
+chemical-potential kernel, and the FP16 form; then every EOS in the FP32 form
+(including the FP32 bulk chemical potential), and the large van der Waals
+striped-wall wetting configuration at a size the oracle covers, with its
+adjoint."""
 import contextlib
 import os
 
@@ -117,20 +118,21 @@ def test_fp64_march_long_chunks_match_oracle(cpmp, name, xc):
     g.close()
 
 
+@pytest.mark.parametrize("xc", [4, 32])
 @pytest.mark.parametrize("form,tol", [("fp32", 2e-4), ("fp16", 2e-2), ("fp32a", 2e-4)])
 @pytest.mark.parametrize("name", list(GEOMS))
-def test_reduced_forms_long_chunks_match_oracle(cpmp, name, form, tol):
-    """fused density with chunks of 32 planes (periodic FP32: the completion folded
-    into the next step's chemical-potential kernel) against the FP64 oracle; the
-    FP32-arithmetic form (FP64 storage) forced onto the march collide, which these
-    lattices would otherwise run one thread per node as the FP64 form does"""
+def test_reduced_forms_long_chunks_match_oracle(cpmp, name, form, tol, xc):
+    """fused density with chunks of 4 and 32 planes (periodic FP32: the completion
+    folded into the next step's chemical-potential kernel) against the FP64 oracle;
+    the FP32-arithmetic form (FP64 storage) forced onto the march collide, which
+    these lattices would otherwise run one thread per node as the FP64 form does"""
     case = GEOMS[name]
-    print(f"[production] {form} {name} xc=32 steps={case['steps']}", flush=True)
+    print(f"[production] {form} {name} xc={xc} steps={case['steps']}", flush=True)
     rho0, rho_o, _, fluid = oracle_run(name)
     env = dict(CPMP_COLLIDE="march") if form == "fp32a" else {}
-    g = device(name, cpmp, form, CPMP_XC=32, **env)
+    g = device(name, cpmp, form, CPMP_XC=xc, **env)
     short = device(name, cpmp, form, CPMP_XC=2, **env)
-    unfused = device(name, cpmp, form, CPMP_XC=32, CPMP_FUSE=0, **env)
+    unfused = device(name, cpmp, form, CPMP_XC=xc, CPMP_FUSE=0, **env)
     if not gc.path_forced():
         assert g.memory() > unfused.memory()   # the fused path
     unfused.close()
@@ -145,7 +147,7 @@ def test_reduced_forms_long_chunks_match_oracle(cpmp, name, form, tol):
     rl, rs = g.get("rho"), short.get("rho")
     err = np.max(np.abs(rl[fluid] - rho_o[fluid])) / np.max(rho_o[fluid])
     d_short = np.max(np.abs(rl[fluid] - rs[fluid])) / np.max(rs[fluid])
-    print(f"[production] {form} {name} xc=32: max |rho - rho_oracle| / max rho {err:.3e}; "
+    print(f"[production] {form} {name} xc={xc}: max |rho - rho_oracle| / max rho {err:.3e}; "
           f"vs chunks of 2 {d_short:.3e}", flush=True)
     assert err < tol
     assert d_short < tol / 4

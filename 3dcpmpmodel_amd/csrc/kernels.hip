@@ -361,7 +361,12 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
     if (!nd.ok) return;
     const long long I = nd.I;
     const Coord c{nd.x, nd.y, nd.z};
-    if (is_solid(P, c.z)) return;
+    if (is_solid(P, c.z)) {   // own slots (never pulled): zeros, as the march collide stores
+        const unsigned Nu = (unsigned)P.N, site = pop_site(P, c.x, c.y, c.z);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fout[(unsigned)q * Nu + site] = TF(0);
+        return;
+    }
     const long long N = P.N;
     int xs[7], ys[7], zs[7];
 #pragma unroll
@@ -779,7 +784,12 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     const int xa = ch * P.xc, xb = min(xa + P.xc, P.nx);
     const int ty = threadIdx.x / MTZ, tz = threadIdx.x % MTZ;
     const int y = y0 + ty, z = z0 + tz;
-    const bool active = (y < P.ny) && (z < P.nz) && (NW || !is_solid(P, z));
+    const bool inr = (y < P.ny) && (z < P.nz);
+    const bool active = inr && (NW || !is_solid(P, z));
+    // The lanes of solid nodes store zeros into their own slots (which are never
+    // pulled) in the same instructions as the fluid lanes of their row, so that a
+    // population row of the tile is written whole: a partial line costs the memory
+    // a read-modify-write (measured 9-10 % of the collide on walled lattices).
 
     // Halo entries of one plane owned by this thread (e = tid + h*MT) and their
     // in-plane offsets; the plane index is added per x.
@@ -907,44 +917,54 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
         store_plane((x - xa + 2 * MH + 1) & 7);
         fetch_plane(x + 2 + MH);   // past the chunk on the last passes (wrapped, unused)
         gather(x + 1, fnext);
-        if (active) {
+        // NW: fluid lanes only; otherwise the solid lanes too, which store zeros
+        // (fluid: constant true under NW, the zero path compiles away)
+        if (NW ? active : inr) {
+            const bool fluid = NW || active;
             int slot[7];
 #pragma unroll
             for (int d = 0; d < 7; ++d) slot[d] = (x - xa + d) & 7;
-            const LdsStencil2<TS, TC> st{s_rm, slot, base};
-            const TC r = (TC)st.at(0, 0, 0).r;
-            TC gmx, gmy, gmz, grx, gry, grz;
-            st.grad(grx, gry, grz, gmx, gmy, gmz);
-            const TC third = TC(1.0 / 3.0);
-            const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
-            TC out[Q], u[3];
-            if (QPR && P.qp_on) {   // QPR = false: the Qp term is off (K1 = K2 = 0), compiled out
-                TC qp[Q];
-                qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
-                collide<TC, UR, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
-            } else {
-                collide<TC, UR, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
-            }
-            if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
-                const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
+            TC out[Q], r = TC(0);
+            if (fluid) {
+                const LdsStencil2<TS, TC> st{s_rm, slot, base};
+                r = (TC)st.at(0, 0, 0).r;
+                TC gmx, gmy, gmz, grx, gry, grz;
+                st.grad(grx, gry, grz, gmx, gmy, gmz);
+                const TC third = TC(1.0 / 3.0);
+                const TC Fx = -gmx * r + grx * third, Fy = -gmy * r + gry * third, Fz = -gmz * r + grz * third;
+                TC u[3];
+                if (QPR && P.qp_on) {   // QPR = false: the Qp term is off (K1 = K2 = 0), compiled out
+                    TC qp[Q];
+                    qp_terms(grx, gry, grz, (TC)P.K1, (TC)P.K2, qp);
+                    collide<TC, UR, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), qp, out, u);
+                } else {
+                    collide<TC, UR, GUO>(f, r, Fx, Fy, Fz, rates<TC>(P), (const TC *)nullptr, out, u);
+                }
+                if (uout) {   // adjoint tape: u, grad mu, grad rho (9 N values)
+                    const TC t9[9] = {u[0], u[1], u[2], gmx, gmy, gmz, grx, gry, grz};
 #pragma unroll
-                for (int a = 0; a < 9; ++a) uout[a * N + I] = (TS)t9[a];
-            }
-            if (MV) {
-                const TC h = TC(0.5) / r;
-                mv[I] = (double)(u[0] + Fx * h);
-                mv[N + I] = (double)(u[1] + Fy * h);
-                mv[2 * N + I] = (double)(u[2] + Fz * h);
+                    for (int a = 0; a < 9; ++a) uout[a * N + I] = (TS)t9[a];
+                }
+                if (MV) {
+                    const TC h = TC(0.5) / r;
+                    mv[I] = (double)(u[0] + Fx * h);
+                    mv[N + I] = (double)(u[1] + Fy * h);
+                    mv[2 * N + I] = (double)(u[2] + Fz * h);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) out[q] = TC(0);
             }
             // aligned stores: the next step pulls these post-collision values
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 // FP16: shifted by the density its puller has in this step (this
-                // node's when bounced back), read from the LDS ring
+                // node's when bounced back), read from the LDS ring; a solid node's
+                // zeros unshifted
                 const bool bounced = (cz(q) < 0 && bb_m) || (cz(q) > 0 && bb_p);
                 TC tref = TC(0);
                 if constexpr (kShifted<TF>)
-                    tref = bounced ? r : (TC)s_rm[slot[3 + cx(q)]][base + cy(q) * MLZ + cz(q)].r;
+                    tref = !fluid ? TC(0) : bounced ? r : (TC)s_rm[slot[3 + cx(q)]][base + cy(q) * MLZ + cz(q)].r;
                 const TF sv = pop_val<TF, TC>(out[q], q, tref);
                 // streaming store: the populations are read again only next step,
                 // after far more data than the caches hold has passed through
@@ -953,9 +973,11 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
                 if constexpr (sizeof(TF) >= 4) __builtin_nontemporal_store(sv, &fout[at]);
                 else fout[at] = sv;
                 if (RS) {
-                    // the value the next step pulls: as stored (and decoded)
+                    // the value the next step pulls: as stored (and decoded); a solid
+                    // node's slots are not pulled and add nothing
                     TC w;
                     w = pop_load<TF, TC>(&sv, 0, q, tref);
+                    if (!fluid) w = TC(0);
                     const int qx = cx(q), qy = cy(q), qz = cz(q);
                     if (qy == 0 && qz == 0) {           // resolved per q at compile time
                         if (qx == 0) acc_cur += w;

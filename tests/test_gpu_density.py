@@ -243,3 +243,29 @@ def test_runs_split_into_calls_are_bitwise_the_same(cpmp, name, form):
         assert np.array_equal(a.get(field), b.get(field)), (form, field)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("collide", ["march", "v1"])
+@pytest.mark.parametrize("form", ["fp64", "fp32", "fp16", "fp32a"])
+def test_solid_populations_read_zero_after_a_step(cpmp, form, collide):
+    """the lanes of solid nodes store zeros into their own slots with the fluid
+    lanes of their row (whole population rows), and the per-node collide does the
+    same: after a step a solid node's 19 values read 0, its fluid neighbours are
+    unaffected (the density stays the sum of the pulled populations)"""
+    old = os.environ.get("CPMP_COLLIDE")
+    os.environ["CPMP_COLLIDE"] = collide
+    try:
+        g = make(cpmp, "tiled_prw_64x16x64", form)
+    finally:
+        if old is None:
+            os.environ.pop("CPMP_COLLIDE", None)
+        else:
+            os.environ["CPMP_COLLIDE"] = old
+    fluid = g.get("type") == 1
+    for n in (1, 2):
+        g.step(n)
+        f = g.get("f").reshape(19, -1)
+        assert np.all(f[:, ~fluid] == 0), (form, collide, n)
+        rho, s = g.get("rho"), f.sum(axis=0)
+        assert np.max(np.abs(rho[fluid] - s[fluid])) < 2e-6 * np.max(np.abs(s[fluid]))
+    g.close()

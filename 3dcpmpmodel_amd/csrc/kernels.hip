@@ -829,7 +829,19 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
         nb.ox[2] = (wrap_off(xg, 1, P.nx) - xg) * (int)plane;
         const TS ref = ref_at<TF>(rref, ((long long)xg * P.ny + ym) * P.nz + zm);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) dst[q] = pop_load<TF, TC>(fin, pull_index(nb, (unsigned)N, q), q, ref);
+        for (int q = 0; q < Q; ++q) {
+            // FP16 and FP64 storage: non-temporal loads (measured 2.3 % and 1-4 % faster
+            // steps at 256^3); FP32 keeps ordinary loads (2 % slower with the hint:
+            // its z-shifted pulls share partial lines with the neighbouring z tile,
+            // which the L2 serves)
+            const unsigned i = pull_index(nb, (unsigned)N, q);
+            if constexpr (sizeof(TF) == 4) {
+                dst[q] = pop_load<TF, TC>(fin, i, q, ref);
+            } else {
+                const TF v = __builtin_nontemporal_load(&fin[i]);
+                dst[q] = pop_load<TF, TC>(&v, 0, q, ref);
+            }
+        }
     };
     // software pipeline: the populations of plane x+1 and the halo of plane x+4
     // are in flight while plane x computes

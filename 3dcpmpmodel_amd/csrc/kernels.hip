@@ -830,12 +830,14 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
         const TS ref = ref_at<TF>(rref, ((long long)xg * P.ny + ym) * P.nz + zm);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            // FP16 and FP64 storage: non-temporal loads (measured 2.3 % and 1-4 % faster
-            // steps at 256^3); FP32 keeps ordinary loads (2 % slower with the hint:
-            // its z-shifted pulls share partial lines with the neighbouring z tile,
-            // which the L2 serves)
+            // Non-temporal loads: a line is read by one workgroup only, except the
+            // partial lines of the z-shifted pulls, which the neighbouring z tile
+            // reads too and the L2 serves.  FP32 keeps ordinary loads for those (all
+            // 19 non-temporal: 2 % slower; the 9 pulls without a z shift: 1 % faster
+            // per step at 256^3); FP16 and FP64 storage take the hint on every pull
+            // (2.3 % and 1-4 % faster steps)
             const unsigned i = pull_index(nb, (unsigned)N, q);
-            if constexpr (sizeof(TF) == 4) {
+            if (sizeof(TF) == 4 && cz(q) != 0) {
                 dst[q] = pop_load<TF, TC>(fin, i, q, ref);
             } else {
                 const TF v = __builtin_nontemporal_load(&fin[i]);

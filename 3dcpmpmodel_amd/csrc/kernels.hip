@@ -361,12 +361,9 @@ __global__ void __launch_bounds__(TPB) k_collide(const DevParams P, const TF *__
     if (!nd.ok) return;
     const long long I = nd.I;
     const Coord c{nd.x, nd.y, nd.z};
-    if (is_solid(P, c.z)) {   // own slots (never pulled): zeros, as the march collide stores
-        const unsigned Nu = (unsigned)P.N, site = pop_site(P, c.x, c.y, c.z);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) fout[(unsigned)q * Nu + site] = TF(0);
-        return;
-    }
+    // a solid node's slots are never pulled: left as they are (storing zeros into them,
+    // as the march collide does for its whole rows, measured 6 % slower here)
+    if (is_solid(P, c.z)) return;
     const long long N = P.N;
     int xs[7], ys[7], zs[7];
 #pragma unroll

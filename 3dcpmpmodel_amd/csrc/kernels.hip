@@ -1055,8 +1055,9 @@ __global__ void __launch_bounds__(MT) k_density_fix(const DevParams P, TS *__res
         pbase[i] = ((((long long)cp * nty + yy) * ntz + zz) * 2 + 1) * AP;
         nbase[i] = ((((long long)cn * nty + yy) * ntz + zz) * 2 + 0) * AP;
     }
-    // every item of this thread at once (all loads in flight, the densities the items
-    // complete among them: 0.7 % off the FP16 step), then the updates
+    // every item of this thread at once (all loads in flight, with 4-byte densities
+    // the densities the items complete among them: 0.7 % off the FP16 step; in FP64
+    // they would cost 20 VGPRs and 2 of 8 waves per SIMD), then the updates
     constexpr int PER = (FIX_ITEMS_MAX + MT - 1) / MT;
     const int items = P.xc * ETILE + 2 * ITILE;
     const long long plane_ring = (long long)nty * ntz * ARING;
@@ -1110,11 +1111,11 @@ __global__ void __launch_bounds__(MT) k_density_fix(const DevParams P, TS *__res
         if (hi) a += cbuf[nbase[4] + (ty + 1) * AZ + tz + 1];
         add[j] = a;
         at[j] = ((long long)(ch * P.xc + k) * P.ny + tyi * MTY + ty) * P.nz + z;
-        old[j] = rho[at[j]];
+        if constexpr (sizeof(TS) == 4) old[j] = rho[at[j]];
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j)
-        if (at[j] >= 0) rho[at[j]] = old[j] + add[j];
+        if (at[j] >= 0) rho[at[j]] = (sizeof(TS) == 4 ? old[j] : rho[at[j]]) + add[j];
 }
 
 // ---------------------------------------------------------------------------

@@ -819,8 +819,34 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     // the in-plane neighbour offsets and wall flags do not change along x
     Nbr nb = make_nbr(P, xa, ym, zm);
     if (NW) nb.sb[0] = nb.sb[2] = false;   // no solid layers: nothing bounces back
+    // FP16 without solid layers: each pull is a per-plane base (uniform) plus a byte
+    // offset fixed along x, and each store likewise, which takes the 64-bit address
+    // arithmetic off the VALU the FP16 collide is bound by (1 % of its step; 19 N
+    // values of 2 bytes fit 32-bit offsets since the context requires 19 N < 2^31).
+    // The memory-bound FP32 and FP64 collides measured no gain.
+    constexpr bool VO = NW && kShifted<TF>;
+    unsigned vo[VO ? Q : 1];
+    if constexpr (VO) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            vo[q] = ((unsigned)q * (unsigned)N + (unsigned)((int)own_site + nb.oyz[(1 - cy(q)) * 3 + 1 - cz(q)])) *
+                    (unsigned)sizeof(TF);
+    }
     auto gather = [&](int xg, TC *dst) {
         xg = wrapc(xg, P.nx);
+        if constexpr (VO) {
+            const char *b0 = (const char *)fin + (size_t)xg * plane * sizeof(TF);
+            const char *bm = (const char *)fin + (size_t)wrap_off(xg, -1, P.nx) * plane * sizeof(TF);
+            const char *bp = (const char *)fin + (size_t)wrap_off(xg, 1, P.nx) * plane * sizeof(TF);
+            const TS ref = ref_at<TF>(rref, ((long long)xg * P.ny + ym) * P.nz + zm);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const char *b = cx(q) > 0 ? bm : cx(q) < 0 ? bp : b0;   // pulled from x - c_x
+                const TF v = __builtin_nontemporal_load((const TF *)(b + vo[q]));
+                dst[q] = pop_load<TF, TC>(&v, 0, q, ref);
+            }
+            return;
+        }
         nb.site = (unsigned)xg * plane + own_site;
         nb.ox[0] = (wrap_off(xg, -1, P.nx) - xg) * (int)plane;
         nb.ox[2] = (wrap_off(xg, 1, P.nx) - xg) * (int)plane;
@@ -981,8 +1007,14 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
                 // after far more data than the caches hold has passed through (a
                 // wave's store covers whole lines in every form: FP16 2 % faster per
                 // step than with ordinary stores)
-                const unsigned at = (unsigned)q * (unsigned)N + (unsigned)x * plane + own_site;
-                __builtin_nontemporal_store(sv, &fout[at]);
+                if constexpr (NW && kShifted<TF>) {
+                    TF *dp = (TF *)((char *)fout + ((size_t)q * (size_t)N + (size_t)x * plane) * sizeof(TF) +
+                                    own_site * (unsigned)sizeof(TF));
+                    __builtin_nontemporal_store(sv, dp);
+                } else {
+                    const unsigned at = (unsigned)q * (unsigned)N + (unsigned)x * plane + own_site;
+                    __builtin_nontemporal_store(sv, &fout[at]);
+                }
                 if (RS) {
                     // the value the next step pulls: as stored (and decoded); a solid
                     // node's slots are not pulled and add nothing

@@ -820,9 +820,10 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     Nbr nb = make_nbr(P, xa, ym, zm);
     if (NW) nb.sb[0] = nb.sb[2] = false;   // no solid layers: nothing bounces back
     // FP16 without solid layers: each pull is a per-plane base (uniform) plus a byte
-    // offset fixed along x, and each store likewise, which takes the 64-bit address
-    // arithmetic off the VALU the FP16 collide is bound by (1 % of its step; 19 N
-    // values of 2 bytes fit 32-bit offsets since the context requires 19 N < 2^31).
+    // offset fixed along x, and each store one base per plane plus a 32-bit offset,
+    // which takes 64-bit address arithmetic and scalar-register spills off the VALU
+    // the FP16 collide is bound by (loads 1 %, stores 3.5 % of its step; 19 N values
+    // of 2 bytes fit 32-bit offsets since the context requires 19 N < 2^31).
     // The memory-bound FP32 and FP64 collides measured no gain.
     constexpr bool VO = NW && kShifted<TF>;
     unsigned vo[VO ? Q : 1];
@@ -1008,8 +1009,8 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
                 // wave's store covers whole lines in every form: FP16 2 % faster per
                 // step than with ordinary stores)
                 if constexpr (NW && kShifted<TF>) {
-                    TF *dp = (TF *)((char *)fout + ((size_t)q * (size_t)N + (size_t)x * plane) * sizeof(TF) +
-                                    own_site * (unsigned)sizeof(TF));
+                    char *sb = (char *)fout + (size_t)x * plane * sizeof(TF);   // uniform per plane
+                    TF *dp = (TF *)(sb + ((unsigned)q * (unsigned)N + own_site) * (unsigned)sizeof(TF));
                     __builtin_nontemporal_store(sv, dp);
                 } else {
                     const unsigned at = (unsigned)q * (unsigned)N + (unsigned)x * plane + own_site;

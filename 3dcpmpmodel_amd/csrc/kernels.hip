@@ -840,10 +840,21 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
             const char *bm = (const char *)fin + (size_t)wrap_off(xg, -1, P.nx) * plane * sizeof(TF);
             const char *bp = (const char *)fin + (size_t)wrap_off(xg, 1, P.nx) * plane * sizeof(TF);
             const TS ref = ref_at<TF>(rref, ((long long)xg * P.ny + ym) * P.nz + zm);
+            // The empty asm makes the offsets values of this plane: otherwise the
+            // compiler hoists their zero extension out of the x loop, keeps 19
+            // 64-bit register pairs and adds each to its base in 64 bits instead of
+            // using the scalar-base form of the load (FP16 step 0.8 % faster).
+            static_assert(Q == 19, "one asm operand per population");
+            unsigned o[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) o[q] = vo[q];
+            asm volatile("" : "+v"(o[0]), "+v"(o[1]), "+v"(o[2]), "+v"(o[3]), "+v"(o[4]), "+v"(o[5]), "+v"(o[6]),
+                         "+v"(o[7]), "+v"(o[8]), "+v"(o[9]), "+v"(o[10]), "+v"(o[11]), "+v"(o[12]), "+v"(o[13]),
+                         "+v"(o[14]), "+v"(o[15]), "+v"(o[16]), "+v"(o[17]), "+v"(o[18]));
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const char *b = cx(q) > 0 ? bm : cx(q) < 0 ? bp : b0;   // pulled from x - c_x
-                const TF v = __builtin_nontemporal_load((const TF *)(b + vo[q]));
+                const TF v = __builtin_nontemporal_load((const TF *)(b + o[q]));
                 dst[q] = pop_load<TF, TC>(&v, 0, q, ref);
             }
             return;

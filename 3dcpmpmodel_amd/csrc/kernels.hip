@@ -822,7 +822,7 @@ __global__ void __launch_bounds__(MT) k_collide_march(const DevParams P, const T
     // FP16 without solid layers: each pull is a per-plane base (uniform) plus a byte
     // offset fixed along x, and each store one base per plane plus a 32-bit offset,
     // which takes 64-bit address arithmetic and scalar-register spills off the VALU
-    // the FP16 collide is bound by (loads 1 %, stores 3.5 % of its step; 19 N values
+    // the FP16 collide is bound by (pulls 1.8 %, stores 3.5 % of its step; 19 N values
     // of 2 bytes fit 32-bit offsets since the context requires 19 N < 2^31).
     // The memory-bound FP32 and FP64 collides measured no gain.
     constexpr bool VO = NW && kShifted<TF>;
